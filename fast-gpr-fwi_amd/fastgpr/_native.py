@@ -1,0 +1,97 @@
+"""ctypes binding of libfastgpr.so (the C ABI declared in include/fastgpr.h).
+
+Replaces the reference's six ``ctypes.cdll.LoadLibrary`` handles and their
+argtypes tables (src/__init__.py:30-221) with one library.  Unlike the
+reference (src/__init__.py:14-18) nothing is compiled on import: the shared
+object must already be built (``fast-gpr-fwi_amd/csrc/build.py``), and a
+missing library is an ImportError -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+# torch first: it loads its bundled HIP/HSA runtime, and libfastgpr.so must
+# bind to that same runtime instance (loading /opt/rocm's copy first would put
+# two HSA runtimes in the process and the second one sees no device).
+import torch  # noqa: F401
+
+_LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
+
+ABI_VERSION = 1
+EXEC_EAGER = 0
+EXEC_GRAPH = 1
+GRAD_REFERENCE = 0
+
+
+class Problem(ctypes.Structure):
+    """Mirror of ``fgpr_problem`` (include/fastgpr.h)."""
+
+    _fields_ = [
+        ("NI", ctypes.c_int32), ("NJ", ctypes.c_int32), ("P", ctypes.c_int32),
+        ("S", ctypes.c_int32), ("nt", ctypes.c_int32), ("nsrc", ctypes.c_int32),
+        ("nrx", ctypes.c_int32), ("pitch", ctypes.c_int32),
+        ("dx", ctypes.c_float), ("dt", ctypes.c_float),
+        ("coef", ctypes.c_void_p), ("pml", ctypes.c_void_p), ("src", ctypes.c_void_p),
+        ("srcscale", ctypes.c_void_p), ("rx", ctypes.c_void_p),
+        ("Ez", ctypes.c_void_p), ("Hx", ctypes.c_void_p), ("Hy", ctypes.c_void_p),
+        ("psi", ctypes.c_void_p), ("traces", ctypes.c_void_p), ("snaps", ctypes.c_void_p),
+        ("resid", ctypes.c_void_p), ("grad_eps", ctypes.c_void_p), ("grad_sig", ctypes.c_void_p),
+        ("ctr", ctypes.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/fastgpr.h declares.
+SIGNATURES = {
+    "fgpr_abi_version": (ctypes.c_int, []),
+    "fgpr_last_error": (ctypes.c_char_p, []),
+    "fgpr_pitch": (ctypes.c_int, [ctypes.c_int]),
+    "fgpr_psi_floats": (ctypes.c_int64, [ctypes.c_int] * 4),
+    "fgpr_validate": (ctypes.c_int, [ctypes.POINTER(Problem)]),
+    "fgpr_cell_coeffs": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "fgpr_forward": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "fgpr_adjoint": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "fgpr_normalise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "fgpr_release_graphs": (ctypes.c_int, []),
+}
+
+
+def _load():
+    if not _LIB_PATH.is_file():
+        raise ImportError(
+            f"libfastgpr.so not found at {_LIB_PATH}; build it with "
+            "`python fast-gpr-fwi_amd/csrc/build.py` (hipcc, gfx950). "
+            "There is no CPU fallback.")
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.fgpr_abi_version() != ABI_VERSION:
+        raise ImportError(f"libfastgpr ABI {lib.fgpr_abi_version()} != expected {ABI_VERSION}")
+    return lib
+
+
+lib = _load()
+LIB_PATH = _LIB_PATH
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise RuntimeError carrying fgpr_last_error() for a non-zero status."""
+    if rc != 0:
+        msg = lib.fgpr_last_error().decode(errors="replace")
+        raise RuntimeError(f"fastgpr{(' ' + what) if what else ''} failed ({rc}): {msg}")
+
+
+def pitch(NJ: int) -> int:
+    return lib.fgpr_pitch(NJ)
+
+
+def psi_floats(NI: int, NJ: int, P: int, S: int) -> int:
+    return lib.fgpr_psi_floats(NI, NJ, P, S)

@@ -1,0 +1,157 @@
+"""Drop-in replacement of the reference's ``src/compute.py`` call surface.
+
+``compute`` keeps the reference signature (compute.py:33-37) and return value
+``(traces (S, nt, nrx) float32, dt)``; ``GPRFWI`` is the autograd Function
+(compute.py:877-942) whose backward returns the normalised reference
+gradients (compute.py:1174-1180) plus the optional TV term
+(compute.py:916-933).  The internals run on libfastgpr (HIP, gfx950).
+
+Extra keyword arguments (all default to the reference behaviour):
+
+* ``exec_mode``   -- "graph" (time loop replayed from captured HIP graphs) or
+                     "eager" (one kernel sequence per step from the host loop).
+* ``graph_chunk`` -- steps per captured graph.
+"""
+from __future__ import annotations
+
+import time
+
+import torch
+from torch import Tensor
+
+from . import grid as _grid
+from . import solver as _solver
+from .grid import FDTDGrid
+
+
+class GPRFWIModel(torch.nn.Module):
+    """nn.Module holding er / se as parameters (compute.py:16-29).  Its
+    forward calls ``compute`` with keyword arguments, which fixes the
+    reference's positional mix-up of ``time_windows`` into ``source``."""
+
+    def __init__(self, se: Tensor, er: Tensor, serequires_grad: bool = False,
+                 errequires_grad: bool = False):
+        super().__init__()
+        self.er = torch.nn.Parameter(er, requires_grad=errequires_grad)
+        self.se = torch.nn.Parameter(se, requires_grad=serequires_grad)
+
+    def forward(self, device, dx, ny, nx, nz, time_windows, source, source_location, source_step,
+                receiver_location, receiver_step, pmlthick, source_apmlitudes, step, mr, nsrc, nrx,
+                freq=None, dt=None, pml_formulation='HORIPML', total_step=None, tv=None):
+        return compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, self.er, self.se,
+                       mr, nsrc, nrx, time_windows=time_windows, source_location=source_location,
+                       source_step=source_step, receiver_location=receiver_location,
+                       receiver_step=receiver_step, pmlthick=pmlthick, freq=freq, dt=dt,
+                       pml_formulation=pml_formulation, total_step=total_step, tv=tv)
+
+
+def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr, nsrc, nrx,
+            time_windows=None, source_location=None, source_step=None,
+            receiver_location=None, receiver_step=None, pmlthick=0, freq=None,
+            dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
+            Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
+            exec_mode="graph", graph_chunk=32):
+    """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
+
+    Same arguments and meaning as compute.py:33-50: ``nx`` is the depth
+    (first axis) and ``ny`` the lateral size, ``step`` the number of shots,
+    er / se / mr are (nx, ny, 1) tensors, locations and steps are metres.
+    se is clamped at 0 and er at 1 (differentiably) before the solve.
+    """
+    se = torch.clamp(se, min=0)
+    er = torch.clamp(er, min=1)
+    G = initialization(device, dx, ny, nx, nz, time_windows, source, source_location,
+                       source_step, receiver_location, receiver_step, pmlthick,
+                       source_apmlitudes, step, er, se, mr, nsrc, nrx, dt, pml_formulation,
+                       freq, total_step, tv, Fixedreceiver, nseg, customsource, customreceiver,
+                       exec_mode=exec_mode, graph_chunk=graph_chunk)
+    rxs = GPRFWI.apply(G, er, se, mr)
+    return rxs, G.dt
+
+
+def initialization(device, dx, ny, nx, nz, time_windows, source, source_location, source_step,
+                   receiver_location, receiver_step, pmlthick, source_apmlitudes, step, er, se,
+                   mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
+                   nseg, customsource=None, customreceiver=None, exec_mode="graph",
+                   graph_chunk=32) -> FDTDGrid:
+    """Build the solve description (compute.py:726-872).
+
+    ``nseg`` (the reference's unfinished segmenting, compute.py:543-553) and
+    ``freq`` (whose filtered wavelet is discarded, compute.py:825-826) are
+    accepted and have no effect, as in the reference.
+    """
+    device = _solver.require_hip(device)
+    if pml_formulation != 'HORIPML':
+        raise ValueError("only the HORIPML formulation is implemented (compute.py:193)")
+    if source_apmlitudes[0] != 'ricker':
+        raise ValueError("only the 'ricker' waveform is implemented (compute.py:393)")
+    if source[1] != 'z':
+        raise ValueError("2-D TMz supports z-polarised sources only")
+    if exec_mode not in ("graph", "eager"):
+        raise ValueError("exec_mode must be 'graph' or 'eager'")
+    NI, NJ = int(nx), int(ny)
+    if int(nz) != 1:
+        raise ValueError("2-D TMz solver: nz must be 1")
+    S = int(step)
+    if tuple(er.shape[:2]) != (NI, NJ):
+        raise ValueError(f"er shape {tuple(er.shape)} does not match (nx, ny) = {(NI, NJ)}")
+    Gdt = _grid.resolve_dt(dx, dt)
+    ts = S if total_step is None else int(total_step)
+    src, rx = _grid.geometry(nsrc, nrx, ts, source_location, source_step, receiver_location,
+                             receiver_step, dx, Fixedreceiver, customsource, customreceiver)
+    if time_windows is None:
+        raise ValueError("time_windows is required")
+    tw = time_windows if isinstance(time_windows, torch.Tensor) else torch.tensor(time_windows)
+    nt = _grid.n_iterations(tw, Gdt)
+    amp, f0 = source_apmlitudes[1], source_apmlitudes[2]
+    w = _grid.ricker(nt, Gdt, amp, f0, tw)
+    srcwaves = torch.zeros((src.shape[1], nt), dtype=torch.float32)
+    if src.shape[1] > 0:
+        srcwaves[0] = w          # one dipole object: only row 0 emits (compute.py:563-564)
+    P = int(pmlthick)
+    pml = _grid.pml_profile(P, dx, Gdt, er.detach(), mr.detach()) if P > 0 else None
+    return FDTDGrid(device=device, NI=NI, NJ=NJ, S=S, nsrc=int(src.shape[1]), nrx=int(rx.shape[1]),
+                    dx=dx, dt=Gdt, iterations=nt, pmlthick=P, source=src, receiver=rx,
+                    srcwaves=srcwaves, pml=pml, tv=tv, exec_mode=exec_mode,
+                    graph_chunk=int(graph_chunk))
+
+
+class GPRFWI(torch.autograd.Function):
+    """Forward returns the Ez traces (S, nt, nrx); backward returns the
+    L2-normalised reference gradients for er / se (compute.py:877-942)."""
+
+    @staticmethod
+    def forward(ctx, G, er, se, mr):
+        ctx.save_for_backward(er, se, mr)
+        ctx.G = G
+        need_grad = er.requires_grad or se.requires_grad
+        traces, batch = _solver.forward(G, er, se, mr, store=need_grad)
+        ctx.batch = batch if need_grad else None
+        if torch.isnan(traces).any():
+            print("⚠️ rxs_gpu contains NaN!")
+        return traces
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        if torch.isnan(grad_output).any():
+            print("⚠️ grad_output contains NaN!")
+        er, se, mr = ctx.saved_tensors
+        G = ctx.G
+        g_er, g_se, _, _ = _solver.adjoint(G, ctx.batch, er, se, mr, grad_output,
+                                           want_er=er.requires_grad, want_se=se.requires_grad)
+        ctx.batch = None
+        if G.tv:
+            from .tv import total_variation
+            t0 = time.time()
+            if er.requires_grad:
+                uer = total_variation(er.clone())
+                g_er += 0.01 * (2 * (er - uer))
+                if torch.isnan(g_er).any():
+                    print("⚠️ epsilon.grad contains NaN!")
+            if se.requires_grad:
+                use = total_variation(se.clone())
+                g_se += 0.01 * (2 * (se - use))
+                if torch.isnan(g_se).any():
+                    print("⚠️ sigma.grad contains NaN!")
+            print(f"Total variation regularization took {time.time() - t0:.4f} seconds.")
+        return None, g_er, g_se, None

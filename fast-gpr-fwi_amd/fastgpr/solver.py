@@ -1,0 +1,153 @@
+"""Device side of one solve: buffers, coefficient set-up and the native
+forward / adjoint sweeps (the reference's ``Forward`` compute.py:542-720 and
+``Backward`` compute.py:1020-1180).
+
+All device memory is allocated here as torch tensors (so the caching
+allocator and any HBM budget see it) and handed to libfastgpr by pointer.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native
+from ._native import check, lib
+
+_MODES = {"eager": _native.EXEC_EAGER, "graph": _native.EXEC_GRAPH}
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_hip(device) -> torch.device:
+    device = torch.device(device)
+    if device.type != "cuda":
+        raise RuntimeError("fastgpr runs on a HIP (ROCm) device only; got device "
+                           f"{device}. There is no CPU path.")
+    if not torch.cuda.is_available():
+        raise RuntimeError("fastgpr needs a ROCm GPU; torch.cuda.is_available() is False")
+    return device
+
+
+def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device) -> torch.Tensor:
+    """5 x (NI+1) x pitch coefficient planes (Ucget / Ucgeta, uc.cu:48, 94)."""
+    NI, NJ = er.shape[0], er.shape[1]
+    pitch = _native.pitch(NJ)
+    e = er.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
+    s = se.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
+    m = mr.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
+    coef = torch.empty((5, NI + 1, pitch), dtype=torch.float32, device=device)
+    check(lib.fgpr_cell_coeffs(_ptr(e), _ptr(s), _ptr(m), NI, NJ, dt32, dx32,
+                               1 if adjoint else 0, _ptr(coef), _stream(device)), "cell_coeffs")
+    return coef
+
+
+class Batch:
+    """Device buffers of one batch of S shots on one grid."""
+
+    def __init__(self, G, store: bool = True):
+        dev = G.device
+        self.G = G
+        self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, G.S, G.iterations
+        self.P = G.pmlthick
+        self.pitch = _native.pitch(self.NJ)
+        self.fields = torch.zeros((3, self.S, self.NI + 1, self.pitch), dtype=torch.float32, device=dev)
+        npsi = max(_native.psi_floats(self.NI, self.NJ, self.P, self.S), 1)
+        self.psi = torch.zeros(npsi, dtype=torch.float32, device=dev)
+        self.pml = (G.pml.to(dev).contiguous() if self.P > 0
+                    else torch.zeros(8, dtype=torch.float32, device=dev))
+        from .grid import device_cells
+        if G.source.shape[0] < self.S or G.receiver.shape[0] < self.S:
+            raise ValueError(f"source/receiver arrays have fewer rows than the {self.S} shots")
+        self.src = device_cells(G.source[: self.S], self.NI, self.NJ, "source").to(dev)
+        self.rx = device_cells(G.receiver[: self.S], self.NI, self.NJ, "receiver").to(dev)
+        self.nsrc = self.src.shape[1]
+        self.nrx = self.rx.shape[1]
+        self.ctr = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.traces = torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev)
+        self.snaps = (torch.empty((self.nt, self.S, self.NI, self.pitch), dtype=torch.float32, device=dev)
+                      if store else None)
+
+    def reset_state(self):
+        self.fields.zero_()
+        self.psi.zero_()
+
+    def problem(self, coef, srcscale=None, resid=None, grad_eps=None, grad_sig=None):
+        G = self.G
+        p = _native.Problem()
+        p.NI, p.NJ, p.P, p.S, p.nt = self.NI, self.NJ, self.P, self.S, self.nt
+        p.nsrc, p.nrx, p.pitch = self.nsrc, self.nrx, self.pitch
+        p.dx, p.dt = G.dx32, G.dt32
+        p.coef = coef.data_ptr()
+        p.pml = self.pml.data_ptr()
+        p.src = self.src.data_ptr()
+        p.srcscale = srcscale.data_ptr() if srcscale is not None else None
+        p.rx = self.rx.data_ptr()
+        p.Ez = self.fields[0].data_ptr()
+        p.Hx = self.fields[1].data_ptr()
+        p.Hy = self.fields[2].data_ptr()
+        p.psi = self.psi.data_ptr()
+        p.traces = self.traces.data_ptr()
+        p.snaps = self.snaps.data_ptr() if self.snaps is not None else None
+        p.resid = resid.data_ptr() if resid is not None else None
+        p.grad_eps = grad_eps.data_ptr() if grad_eps is not None else None
+        p.grad_sig = grad_sig.data_ptr() if grad_sig is not None else None
+        p.ctr = self.ctr.data_ptr()
+        return p
+
+
+def source_scale(G) -> torch.Tensor:
+    """w[n]*dl/(dx*dy*dz) per source row in float32, the expression of
+    sourcereceiver.cu:73-75 with dl = dy = dz = dx (compute.py:571, 693)."""
+    d = torch.tensor(G.dx32, dtype=torch.float32)
+    w = G.srcwaves.to(torch.float32)
+    return ((w * d) / ((d * d) * d)).contiguous()
+
+
+def forward(G, er, se, mr, store: bool = True):
+    """Forward sweep over all nt steps.  Returns (traces (S, nt, nrx), batch)."""
+    dev = G.device
+    se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
+    coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
+    b = Batch(G, store=store)
+    scale = source_scale(G).to(dev)
+    p = b.problem(coef, srcscale=scale)
+    check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], G.graph_chunk,
+                           _stream(dev)), "forward")
+    return b.traces, b
+
+
+def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
+    """Reference-mode adjoint sweep (compute.py:1020-1180).
+
+    Returns (grad_er, grad_se) as float32 (NI, NJ, 1) normalised by their L2
+    norms, None where not requested, plus the raw FP64 sums.
+    """
+    dev = G.device
+    coef = cell_coeffs(er, se, mr, G.dt32, G.dx32, True, dev)      # Ucgeta
+    b.reset_state()
+    resid = grad_output.detach().to(device=dev, dtype=torch.float32).contiguous()
+    if resid.shape != (b.S, b.nt, b.nrx):
+        raise ValueError(f"grad_output shape {tuple(resid.shape)} != {(b.S, b.nt, b.nrx)}")
+    NI, NJ = b.NI, b.NJ
+    ge = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    gs = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
+    if b.nt >= 2:
+        check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 1, _native.GRAD_REFERENCE,
+                               _MODES[G.exec_mode], G.graph_chunk, _stream(dev)), "adjoint")
+    out = []
+    for want, g in ((want_er, ge), (want_se, gs)):
+        if not want:
+            out.append(None)
+            continue
+        o = torch.empty((NI, NJ, 1), dtype=torch.float32, device=dev)
+        check(lib.fgpr_normalise(_ptr(g), NI * NJ, 1.0, _ptr(o), _stream(dev)), "normalise")
+        out.append(o)
+    return out[0], out[1], ge, gs

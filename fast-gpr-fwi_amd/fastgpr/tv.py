@@ -1,0 +1,61 @@
+"""Total-variation denoiser used as the prior inside ``GPRFWI.backward``
+(reference: src/tv.py:135-217, called at compute.py:916-933).
+
+ADMM for  min_X  |D X|_1 + 1/(2 lamda) |X - Y|^2  on the max-normalised
+image, zero-padded by one cell, with periodic forward differences D and
+soft-thresholding of the split variables.  The fixed-point sweep, defaults
+(lamda = 0.02, rho = 1, 500 iterations, stop when ||X - X_prev|| < 1e-5)
+and the float32 working precision are those of the reference.
+"""
+from __future__ import annotations
+
+import torch
+
+
+def _dt(z: torch.Tensor, dim: int) -> torch.Tensor:
+    """Adjoint of the periodic forward difference along ``dim``: z - z[+1]."""
+    return z - torch.roll(z, shifts=-1, dims=dim)
+
+
+def _d(x: torch.Tensor, dim: int) -> torch.Tensor:
+    """Periodic backward difference along ``dim``: x - x[-1]."""
+    return x - torch.roll(x, shifts=1, dims=dim)
+
+
+def _shrink(t: torch.Tensor, thr: float) -> torch.Tensor:
+    return torch.sign(t) * torch.clamp(torch.abs(t) - thr, min=0)
+
+
+def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
+                    num_iter: int = 500, tol: float = 1e-5) -> torch.Tensor:
+    """Denoise a (M, N, 1) or (M, N) model; returns (M, N, 1).
+
+    Like the reference, the trailing singleton axis of ``data`` is squeezed
+    in place.
+    """
+    data.squeeze_(-1)
+    dmax = data.max()
+    y = data / (dmax + 1e-6)
+    M, N = y.shape
+    X = torch.zeros((M + 2, N + 2), device=data.device)
+    X[1:-1, 1:-1] = y
+    Y = X.clone()
+    Zx = torch.zeros_like(X)
+    Zy = torch.zeros_like(X)
+    Ux = torch.zeros_like(X)
+    Uy = torch.zeros_like(X)
+    lr = lamda * rho
+    for _ in range(num_iter):
+        X_prev = X.clone()
+        rhs = Y + lr * (_dt(Zx, 1) + _dt(Zy, 0)) - lamda * (_dt(Ux, 1) + _dt(Uy, 0))
+        nb = X[:-2, 1:-1] + X[2:, 1:-1] + X[1:-1, :-2] + X[1:-1, 2:]
+        X[1:-1, 1:-1] = (nb * lr + rhs[1:-1, 1:-1]) / (1 + 4 * lr)
+        dxX = _d(X, 1)
+        dyX = _d(X, 0)
+        Zx = _shrink((Ux + rho * dxX) / rho, 1 / rho)
+        Zy = _shrink((Uy + rho * dyX) / rho, 1 / rho)
+        Ux += rho * (dxX - Zx)
+        Uy += rho * (dyX - Zy)
+        if torch.norm(X - X_prev) < tol:
+            break
+    return (X[1:-1, 1:-1] * dmax).unsqueeze(-1)

@@ -1,0 +1,158 @@
+/*
+ * fastgpr.h -- C ABI of libfastgpr.so, the MI355X (gfx950) 2-D TMz FDTD
+ * forward / adjoint solver with CPML, batched over shots.
+ *
+ * Plain C: pointers, sizes and one descriptor struct.  Every pointer inside
+ * ``fgpr_problem`` is a DEVICE pointer owned by the caller (the Python host
+ * allocates them as torch tensors); the library allocates no device memory
+ * except captured HIP graphs.  Every call is asynchronous on the HIP stream it
+ * is given (``stream`` = hipStream_t, NULL = default stream), returns 0 on
+ * success and a negative code on error, and leaves a message readable through
+ * fgpr_last_error() on the calling thread.
+ *
+ * Index conventions follow the reference: ``NI`` = G.nx (depth, first axis,
+ * index i), ``NJ`` = G.ny (lateral, second axis, index j).  Fields are padded
+ * to (NI+1) x (NJ+1) like the reference (compute.py:499-518) but the dead
+ * k = 1 plane is gone and rows are padded to ``pitch`` floats
+ * (fgpr_pitch(NJ)) for aligned, coalesced rows.
+ *
+ * The reference binds six CUDA libraries through ctypes (src/__init__.py:30-221)
+ * and calls 12 launchers per time step.  This ABI replaces them with:
+ *
+ *   fgpr_cell_coeffs      <- Ucget  (uc.cu:48-55)  and Ucgeta (uc.cu:94-101)
+ *   fgpr_forward          <- per step: launch_store_outputs (sourcereceiver.cu:40),
+ *                            h_fields_updates (fields_updates_gpu.cu:80),
+ *                            order1_{xminus,yminus,xplus,yplus} of pml_updates_h.cu
+ *                            (:84,:314,:196,:431), e_fields_updates
+ *                            (fields_updates_gpu.cu:35), order1_* of
+ *                            pml_updates_e.cu (:80,:290,:173,:405),
+ *                            update_hertzian_dipole (sourcereceiver.cu:89) and the
+ *                            snapshot copy (compute.py:708-712); the time loop
+ *                            of compute.py:574-712 runs inside the call.
+ *   fgpr_adjoint          <- per step: back_source (back.cu:39), e_fields_updates,
+ *                            order1_* (E), h_fields_updates, order1_* (H) and the
+ *                            gradient accumulation of compute.py:1167-1172;
+ *                            the reverse loop of compute.py:1048-1172.
+ *   fgpr_normalise        <- grad / grad.norm() (compute.py:1174-1180).
+ */
+#ifndef FASTGPR_H
+#define FASTGPR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGPR_ABI_VERSION 1
+
+/* Error codes. */
+#define FGPR_OK 0
+#define FGPR_EINVAL (-1)   /* bad argument / inconsistent descriptor */
+#define FGPR_EHIP (-2)     /* HIP runtime error */
+
+/* Execution modes for fgpr_forward / fgpr_adjoint. */
+#define FGPR_EXEC_EAGER 0  /* one kernel sequence per time step, launched from the host loop */
+#define FGPR_EXEC_GRAPH 1  /* time loop replayed from captured HIP graphs of `chunk` steps */
+
+/* Gradient modes for fgpr_adjoint. */
+#define FGPR_GRAD_REFERENCE 0  /* reference's approximate gradient (compute.py:1167-1172) */
+
+/*
+ * One batch of S shots sharing one model and one grid.
+ *
+ * Layouts (all row-major, `pitch` = fgpr_pitch(NJ) floats per row):
+ *   coef      5 planes of (NI+1)*pitch floats: uE0, uE1, uE4, uH1, uH4.
+ *             Zero on i = NI, j = NJ and the pitch padding (uc.cu:19).
+ *             uH0 is identically 1 inside and 0 on the padding, so it is
+ *             not stored.
+ *   pml       8*P floats: ERA, ERB, ERE, ERF, HRA, HRB, HRE, HRF (compute.py:211-222).
+ *   src       S*nsrc*2 int32 cell indices (i, j); i < 0 marks a dead source.
+ *   srcscale  nsrc*nt floats: w[n]*dl/(dx*dy*dz) (sourcereceiver.cu:73-75).
+ *   rx        S*nrx*2 int32 cell indices (i, j); i < 0 marks a dead receiver.
+ *   Ez,Hx,Hy  S*(NI+1)*pitch floats each.
+ *   psi       fgpr_psi_floats(NI, NJ, P, S) floats, 8 slabs in this order:
+ *             E-x0, E-xmax, H-x0, H-xmax  (each S*P*NJ, element [s][q][j]),
+ *             E-y0, E-ymax, H-y0, H-ymax  (each S*NI*P, element [s][i][q]).
+ *   traces    S*nt*nrx floats, element [s][n][r]        (forward output)
+ *   snaps     nt*S*NI*pitch floats, element [n][s][i][j] (forward output,
+ *             adjoint input); NULL = do not store.
+ *   resid     S*nt*nrx floats: dL/dtraces (adjoint input, back.cu:30).
+ *   grad_eps, grad_sig   NI*NJ doubles, accumulated (+=) by fgpr_adjoint.
+ *   ctr       one device int32 used as the step counter of captured graphs.
+ */
+typedef struct fgpr_problem {
+    int32_t NI, NJ, P, S, nt, nsrc, nrx, pitch;
+    float dx, dt;
+    const float* coef;
+    const float* pml;
+    const int32_t* src;
+    const float* srcscale;
+    const int32_t* rx;
+    float* Ez;
+    float* Hx;
+    float* Hy;
+    float* psi;
+    float* traces;
+    float* snaps;
+    const float* resid;
+    double* grad_eps;
+    double* grad_sig;
+    int32_t* ctr;
+} fgpr_problem;
+
+/* ABI version (FGPR_ABI_VERSION). */
+int fgpr_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fgpr_last_error(void);
+
+/* Row pitch, in floats, of every field / coefficient plane for NJ columns. */
+int fgpr_pitch(int NJ);
+
+/* Number of floats of the psi workspace for one batch. */
+int64_t fgpr_psi_floats(int NI, int NJ, int P, int S);
+
+/* Check a descriptor without launching anything (0 = consistent). */
+int fgpr_validate(const fgpr_problem* p);
+
+/*
+ * Per-cell update coefficients from compact (NI, NJ) float er, se, mr into
+ * the 5-plane `coef` layout above.  adjoint = 0 reproduces Ucget
+ * (uc.cu:11-55), adjoint = 1 reproduces Ucgeta (uc.cu:60-101).
+ */
+int fgpr_cell_coeffs(const float* er, const float* se, const float* mr,
+                     int NI, int NJ, float dt, float dx, int adjoint,
+                     float* coef, void* stream);
+
+/*
+ * Forward steps n = n0 .. n1-1 (compute.py:574-712).  Fields and psi must be
+ * zero before step 0 (the caller clears them).  mode = FGPR_EXEC_EAGER or
+ * FGPR_EXEC_GRAPH; chunk = steps per captured graph (ignored when eager).
+ */
+int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, void* stream);
+
+/*
+ * Adjoint steps n = n_hi down to n_lo (inclusive, n_lo >= 1), reference order
+ * (compute.py:1048-1172): inject -resid at the receivers, E + E-CPML, H +
+ * H-CPML, gradient accumulation into grad_eps / grad_sig (FP64, summed over
+ * shots).  grad_sig is accumulated without its constant dt factor.
+ */
+int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int mode,
+                 int chunk, void* stream);
+
+/*
+ * out[k] = (float)(scale * g[k] / ||g||_2) over n doubles (compute.py:1174-1180).
+ * Uses one deterministic single-workgroup reduction.
+ */
+int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* stream);
+
+/* Release every HIP graph the library captured (also done at unload). */
+int fgpr_release_graphs(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTGPR_H */

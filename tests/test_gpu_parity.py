@@ -1,0 +1,90 @@
+"""Parity of the HIP path (through fastgpr.compute -> libfastgpr C ABI)
+against the oracle on identical inputs.
+
+Tolerances:
+* traces: <= 1e-5 relative L2 against the reference FP32 result (the oracle's
+  FP32 mode, which follows the reference's FP32 operation order);
+* gradients: <= 1e-4 relative L2 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_l2, run_oracle, run_product
+
+pytestmark = pytest.mark.gpu
+
+TRACE_TOL_FP32 = 1e-5
+TRACE_TOL_FP64 = 1e-4
+GRAD_TOL_FP64 = 1e-4
+
+
+def _resid(wl, seed=11):
+    from helpers import host_grid
+    nt, _ = host_grid(wl)
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((wl.S, nt, wl.rx.shape[1])).astype(np.float32)
+
+
+def test_small_forward_matches_oracle(hip_device):
+    from fastgpr.workloads import small_case
+    wl = small_case()
+    print("case", wl.name, "S", wl.S, "nt", wl.nt, flush=True)
+    tr, _ = run_product(wl, hip_device)
+    o32, _ = run_oracle(wl, np.float32)
+    o64, _ = run_oracle(wl, np.float64)
+    e32, e64 = rel_l2(tr, o32), rel_l2(tr, o64)
+    print(f"trace rel-L2 vs fp32 oracle {e32:.3e}, vs fp64 {e64:.3e}")
+    assert np.abs(o64).max() > 0
+    assert e32 <= TRACE_TOL_FP32
+    assert e64 <= TRACE_TOL_FP64
+
+
+def test_graph_and_eager_are_bitwise_equal(hip_device):
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=130)
+    a, _ = run_product(wl, hip_device, exec_mode="graph", graph_chunk=16)
+    b, _ = run_product(wl, hip_device, exec_mode="eager")
+    assert np.array_equal(a, b)
+
+
+def test_small_gradient_matches_oracle(hip_device):
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=240)
+    resid = _resid(wl)
+    _, g = run_product(wl, hip_device, resid=resid)
+    _, o = run_oracle(wl, np.float64, resid=resid)
+    for k in ("er", "se"):
+        e = rel_l2(g[k], o[k])
+        print(f"grad {k} rel-L2 vs fp64 oracle {e:.3e}")
+        assert np.isfinite(g[k]).all()
+        assert abs(np.linalg.norm(g[k]) - 1.0) < 1e-5
+        assert e <= GRAD_TOL_FP64
+
+
+def test_single_parameter_gradient(hip_device):
+    """Only er requires grad: se gets no gradient (compute.py:1177-1178)."""
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=120)
+    _, g = run_product(wl, hip_device, resid=_resid(wl), want=("er",))
+    assert "se" not in g and "er" in g
+
+
+def test_c1_traces_and_gradient(hip_device):
+    """Full C1 configuration: 200x100, 20-cell CPML, 1500 steps; residual =
+    traces of the smoothed initial model minus the true-model traces."""
+    from fastgpr.workloads import c1_plain
+    wl = c1_plain()
+    obs, _ = run_product(wl, hip_device)
+    o32, _ = run_oracle(wl, np.float32)
+    e32 = rel_l2(obs, o32)
+    print(f"C1 trace rel-L2 vs fp32 oracle {e32:.3e}")
+    assert e32 <= TRACE_TOL_FP32
+    init = (wl.er0, wl.se0)
+    pred, _ = run_product(wl, hip_device, model=init)
+    resid = (pred - obs).astype(np.float32)
+    _, g = run_product(wl, hip_device, resid=resid, model=init)
+    _, o = run_oracle(wl, np.float64, resid=resid, model=init)
+    for k in ("er", "se"):
+        e = rel_l2(g[k], o[k])
+        print(f"C1 grad {k} rel-L2 vs fp64 oracle {e:.3e}")
+        assert e <= GRAD_TOL_FP64

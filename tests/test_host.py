@@ -1,0 +1,100 @@
+"""Host-side set-up of the product (fastgpr.grid, CPU torch) against the
+oracle and known answers; argument handling that needs no GPU."""
+import numpy as np
+import pytest
+import torch
+
+from fastgpr import grid as g
+from oracle import fdtd_oracle as O
+
+
+def test_dt_rule_and_step_count():
+    assert g.cfl_dt(0.05) == pytest.approx(O.cfl_dt(0.05), rel=1e-15)
+    dt_t = g.cfl_dt(torch.tensor(0.05))             # float32 tensor in, float32 tensor out
+    assert isinstance(dt_t, torch.Tensor) and dt_t.dtype == torch.float32
+    assert g.resolve_dt(0.05, 1e-10) == 1e-10
+    assert g.resolve_dt(0.05, 1.0) == g.cfl_dt(0.05)
+    assert g.n_iterations(torch.tensor(1e-7), 1e-10) == 1001      # example.ipynb
+    assert g.n_iterations(1e-7, 1e-10) == O.n_iterations(1e-7, 1e-10)
+
+
+@pytest.mark.parametrize("freq,dt", [(1e8, 1e-10), (2e8, 5.3e-11), (4e8, 2.1e-11)])
+def test_ricker_matches_oracle_fp32(freq, dt):
+    nt = 800
+    tw = (nt - 1.5) * dt
+    w = g.ricker(nt, dt, 1, freq, torch.tensor(tw)).numpy()
+    o = O.ricker(nt, dt, freq, 1.0, t_stop=tw, dtype=np.float32)
+    o64 = O.ricker(nt, dt, freq, 1.0, t_stop=tw)
+    assert w.dtype == np.float32
+    np.testing.assert_allclose(w, o, atol=2e-6, rtol=0)
+    np.testing.assert_allclose(w, o64, atol=2e-6, rtol=0)
+    assert w[-1] == 0.0          # outside [0, time_window]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("dx", [0.05, "tensor"])
+def test_pml_profile_matches_oracle(dtype, dx):
+    rng = np.random.default_rng(0)
+    er = torch.tensor(rng.uniform(3, 9, (30, 24, 1)), dtype=dtype)
+    mr = torch.ones((30, 24, 1), dtype=dtype)
+    d = torch.tensor(0.05) if dx == "tensor" else 0.05
+    pro = g.pml_profile(10, d, 1e-10, er, mr).numpy()
+    o = O.cpml_profile(10, 0.05, 1e-10, er[0, :, 0].numpy(), np.ones(24))
+    keys = ["ERA", "ERB", "ERE", "ERF", "HRA", "HRB", "HRE", "HRF"]
+    assert pro.shape == (8, 10) and pro.dtype == np.float32
+    for r, k in enumerate(keys):
+        np.testing.assert_allclose(pro[r], o[k], rtol=2e-6, atol=1e-7, err_msg=k)
+
+
+def test_geometry_notebook_known_answer():
+    """example.ipynb acquisition: 40 shots, sources at (0.5+0.25 s, 0.5) m,
+    200 fixed receivers at (0.5+0.05 r, 5.5) m, dx = 0.05 m."""
+    src, rx = g.geometry(1, 200, 40, torch.tensor([0.5, 0.50, 0]), torch.tensor([0.25, 0, 0]),
+                         torch.tensor([0.5, 5.50, 0]), torch.tensor([0.05, 0, 0]),
+                         torch.tensor(0.05), Fixedreceiver=True)
+    assert src.dtype == torch.int32 and src.shape == (40, 1, 3)
+    np.testing.assert_array_equal(src[:, 0, 0].numpy(), 10 + 5 * np.arange(40))
+    np.testing.assert_array_equal(src[:, 0, 1].numpy(), np.full(40, 10))
+    np.testing.assert_array_equal(src[:, 0, 2].numpy(), np.zeros(40))
+    assert rx.shape == (40, 200, 3)
+    np.testing.assert_array_equal(rx[:, :, 0].numpy(), np.tile(10 + np.arange(200), (40, 1)))
+    np.testing.assert_array_equal(rx[:, :, 1].numpy(), np.full((40, 200), 110))
+
+
+def test_geometry_rounds_half_to_even():
+    # positions 1.5, 2.5, 3.5 cells (exact in float32 with dx = 0.5) -> 2, 2, 4
+    src, _ = g.geometry(3, 1, 1, torch.tensor([0.75, 0.0, 0.0]), torch.tensor([0.5, 0.0, 0.0]),
+                        torch.zeros(3), torch.zeros(3), 0.5)
+    np.testing.assert_array_equal(src[0, :, 0].numpy(), [2, 2, 4])
+
+
+def test_device_cells_dead_plane_and_bounds():
+    idx = torch.tensor([[[3, 4, 0], [5, 6, 1]]], dtype=torch.int32)
+    out = g.device_cells(idx, 10, 10, "receiver")
+    np.testing.assert_array_equal(out.numpy(), [[[3, 4], [-1, 6]]])
+    with pytest.raises(ValueError):
+        g.device_cells(torch.tensor([[[11, 0, 0]]]), 10, 10, "source")
+    with pytest.raises(ValueError):
+        g.device_cells(torch.tensor([[[0, -1, 0]]]), 10, 10, "source")
+
+
+def test_source_scale_matches_oracle_bitwise():
+    from fastgpr.solver import source_scale
+    w = g.ricker(300, 1e-10, 1, 1e8, torch.tensor(3e-8))
+    G = type("G", (), {})()
+    G.dx32 = float(np.float32(0.05))
+    G.srcwaves = w[None, :]
+    s = source_scale(G).numpy()
+    o = O.source_scale(w.numpy()[None, :], 0.05, np.float32)
+    np.testing.assert_array_equal(s, o)
+
+
+def test_compute_refuses_cpu_device():
+    import fastgpr
+    er = torch.full((30, 30, 1), 4.0)
+    with pytest.raises(RuntimeError, match="HIP"):
+        fastgpr.compute(torch.device("cpu"), 0.05, 30, 30, 1, ["hertzian", "z"],
+                        ["ricker", 1, 1e8], 1, er, er * 0, er * 0 + 1, 1, 1,
+                        time_windows=1e-8, pmlthick=5,
+                        customsource=torch.tensor([[[15, 15, 0]]]),
+                        customreceiver=torch.tensor([[[10, 10, 0]]]))
