@@ -375,6 +375,318 @@ __global__ void __launch_bounds__(1024) k_normalise(const double* __restrict__ g
 }
 
 // ---------------------------------------------------------------------------
+// Time-blocked kernels.
+//
+// A workgroup owns a TI x TJ tile of one shot and advances it up to K steps
+// per launch.  It loads the tile plus a halo -- a region of RI rows x 64
+// columns -- of Ez, Hx, Hy and the CPML memory variables from the "in"
+// buffers into LDS / registers, runs the steps in LDS, and writes the tile
+// centre to the "out" buffers (ping-pong, so neighbouring workgroups never
+// read cells another one is writing).  Each step the region's interior edges
+// go stale by one more cell; the halo of K cells on the low side and K + 1 on
+// the high side (H reads Ez at +1) absorbs that.  Domain edges are exact: the
+// region never extends past them and cells outside the domain read as zero.
+// Per-model coefficients live in registers for the whole launch; x-slab CPML
+// coefficients come from an LDS table, y-slab ones from registers (a thread
+// owns one column).
+constexpr int kTBCols = 64;
+constexpr int kPmax = 128;
+
+struct TBArgs {
+    Dev d;
+    const float* Ez_in;
+    const float* Hx_in;
+    const float* Hy_in;
+    const float* psi_in;
+    float* Ez_out;
+    float* Hx_out;
+    float* Hy_out;
+    float* psi_out;
+    int ti_count, tj_count;
+    const int* rx_ptr;   // CSR over S*ntiles: receivers sampled (forward) / injected (adjoint)
+    const int* rx_idx;   // entries s*nrx + r
+    double* gshot;       // adjoint: 2*S*NI*NJ per-shot partial gradients
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical ids so that neighbouring tiles (and the shots
+// of one tile, which share coefficients) meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int nwg) {
+    const int q = nwg / 8, r = nwg % 8;
+    const int x = b % 8, t = b / 8;
+    return x * q + (x < r ? x : r) + t;
+}
+
+template <int RI, int NT, int K, bool ADJ>
+__global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nsteps) {
+    constexpr int W = NT / 64;
+    constexpr int CPT = RI / W;
+    constexpr int TI = RI - 2 * K - 1;
+    constexpr int TJ = kTBCols - 2 * K - 1;
+    static_assert(RI % W == 0 && TI > 0 && TJ > 0, "bad time-blocking geometry");
+    const Dev& d = a.d;
+    const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
+    const int n0 = step_index(d, koff, dir);
+
+    const int ntiles = a.ti_count * a.tj_count;
+    const int L = xcd_remap(blockIdx.x, ntiles * d.S);
+    const int s = L % d.S;
+    const int tile = L / d.S;
+    const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
+    const int gi0 = ti * TI - K, gj0 = tj * TJ - K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int gj = gj0 + lane;
+
+    __shared__ float sEz[RI * 64 + 1];
+    __shared__ float sHx[RI * 64 + 1];
+    __shared__ float sHy[RI * 64 + 1];
+    __shared__ float sP[8 * kPmax];
+
+    for (int e = threadIdx.x; e < 8 * P; e += NT) sP[e] = d.pml[e];
+    if (threadIdx.x == 0) { sEz[RI * 64] = 0.f; sHx[RI * 64] = 0.f; sHy[RI * 64] = 0.f; }
+
+    const size_t pl = plane(d);
+    const size_t shot_off = (size_t)s * pl;
+    const bool colx = (gj >= 0 && gj <= NJ);
+    const bool colact = (gj >= 0 && gj < NJ);
+
+    // y-slab CPML membership depends on the column only.
+    int yqh = -1, yqe = -1, ykh = HY0, yke = EY0;
+    if (P > 0 && colact) {
+        if (gj < P) { yqh = P - 1 - gj; ykh = HY0; }
+        else if (gj >= NJ - P) { yqh = gj - (NJ - P); ykh = HYM; }
+        if (gj >= 1 && gj <= P) { yqe = P - gj; yke = EY0; }
+        else if (gj >= NJ - P) { yqe = gj - (NJ - P); yke = EYM; }
+    }
+    float yh[4] = {0.f, 0.f, 0.f, 0.f}, ye[4] = {0.f, 0.f, 0.f, 0.f};
+    if (yqh >= 0) {
+        yh[0] = d.pml[4 * P + yqh] - 1.0f; yh[1] = d.pml[5 * P + yqh];
+        yh[2] = d.pml[6 * P + yqh]; yh[3] = d.pml[7 * P + yqh];
+    }
+    if (yqe >= 0) {
+        ye[0] = d.pml[yqe] - 1.0f; ye[1] = d.pml[P + yqe];
+        ye[2] = d.pml[2 * P + yqe]; ye[3] = d.pml[3 * P + yqe];
+    }
+
+    float cE0[CPT], cE1[CPT], cE4[CPT], cH1[CPT], cH4[CPT];
+    float pHx[CPT], pHy[CPT], pEx[CPT], pEy[CPT];
+    int xqh[CPT], xqe[CPT];
+    unsigned act = 0u;   // bit m: cell m is updated (0 <= i < NI, 0 <= j < NJ)
+
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int r = w + W * m, gi = gi0 + r;
+        const int o = r * 64 + lane;
+        const bool ex = colx && gi >= 0 && gi <= NI;
+        const size_t g = shot_off + (size_t)gi * pitch + gj;
+        sEz[o] = ex ? a.Ez_in[g] : 0.f;
+        sHx[o] = ex ? a.Hx_in[g] : 0.f;
+        sHy[o] = ex ? a.Hy_in[g] : 0.f;
+        const size_t c = (size_t)gi * pitch + gj;
+        cE0[m] = ex ? d.coef[c] : 0.f;
+        cE1[m] = ex ? d.coef[pl + c] : 0.f;
+        cE4[m] = ex ? d.coef[2 * pl + c] : 0.f;
+        cH1[m] = ex ? d.coef[3 * pl + c] : 0.f;
+        cH4[m] = ex ? d.coef[4 * pl + c] : 0.f;
+        const bool ac = colact && gi >= 0 && gi < NI;
+        if (ac) act |= 1u << m;
+        xqh[m] = -1;
+        xqe[m] = -1;
+        if (ac && P > 0) {
+            if (gi < P) xqh[m] = P - 1 - gi;
+            else if (gi >= NI - P) xqh[m] = gi - (NI - P);
+            if (gi >= 1 && gi <= P) xqe[m] = P - gi;
+            else if (gi >= NI - P) xqe[m] = gi - (NI - P);
+        }
+        const float* pin = a.psi_in;
+        pHx[m] = xqh[m] >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xslab_floats(d) +
+                                   ((size_t)s * P + xqh[m]) * NJ + gj] : 0.f;
+        pEx[m] = xqe[m] >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xslab_floats(d) +
+                                   ((size_t)s * P + xqe[m]) * NJ + gj] : 0.f;
+        pHy[m] = (ac && yqh >= 0) ? pin[4 * xslab_floats(d) + (size_t)(ykh - 4) * yslab_floats(d) +
+                                        ((size_t)s * NI + gi) * P + yqh] : 0.f;
+        pEy[m] = (ac && yqe >= 0) ? pin[4 * xslab_floats(d) + (size_t)(yke - 4) * yslab_floats(d) +
+                                        ((size_t)s * NI + gi) * P + yqe] : 0.f;
+    }
+    const int lst = a.rx_ptr ? a.rx_ptr[s * ntiles + tile] : 0;
+    const int lend = a.rx_ptr ? a.rx_ptr[s * ntiles + tile + 1] : 0;
+    const float dx = d.dx;
+    double ae[CPT], as_[CPT];
+    float fnext[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; fnext[m] = 0.f; }
+    __syncthreads();
+
+    // ---- half-steps (shared by forward and adjoint) ----
+    auto h_half = [&]() {
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            if (!(act & (1u << m))) continue;
+            const int r = w + W * m, gi = gi0 + r;
+            const int o = r * 64 + lane;
+            const float ez = sEz[o];
+            const float ezj = sEz[o + 1];
+            const float ezi = (r + 1 < RI) ? sEz[o + 64] : 0.f;
+            if (gi >= 1 || yqh >= 0) {
+                float hx = sHx[o];
+                if (gi >= 1) hx = hx - cH1[m] * (ezj - ez);
+                if (yqh >= 0) {
+                    const float dd = (ezj - ez) / dx;
+                    hx = hx - cH4[m] * (yh[0] * dd + yh[1] * pHy[m]);
+                    pHy[m] = yh[2] * pHy[m] - yh[3] * dd;
+                }
+                sHx[o] = hx;
+            }
+            if (gj >= 1 || xqh[m] >= 0) {
+                float hy = sHy[o];
+                if (gj >= 1) hy = hy + cH1[m] * (ezi - ez);
+                if (xqh[m] >= 0) {
+                    const int q = xqh[m];
+                    const float dd = (ezi - ez) / dx;
+                    hy = hy + cH4[m] * ((sP[4 * P + q] - 1.0f) * dd + sP[5 * P + q] * pHx[m]);
+                    pHx[m] = sP[6 * P + q] * pHx[m] - sP[7 * P + q] * dd;
+                }
+                sHy[o] = hy;
+            }
+        }
+    };
+
+    // E update of cell m (main + CPML in slab order x0, y0, xmax, ymax).
+    auto e_cell = [&](int m, int r, int gi, int o) -> float {
+        float ez = sEz[o];
+        const float hy = sHy[o];
+        const float hx = sHx[o];
+        const float hyim = (gi >= 1 && r >= 1) ? sHy[o - 64] : 0.f;
+        const float hxjm = (gj >= 1 && lane >= 1) ? sHx[o - 1] : 0.f;
+        if (gi >= 1 && gj >= 1) ez = (cE0[m] * ez + cE1[m] * (hy - hyim)) - cE1[m] * (hx - hxjm);
+        const int q = xqe[m];
+        const bool xminus = gi <= P;
+        float dxd = 0.f, dyd = 0.f;
+        if (q >= 0) dxd = (hy - hyim) / dx;
+        if (yqe >= 0) dyd = (hx - hxjm) / dx;
+        if (q >= 0 && xminus) {
+            ez = ez + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
+            pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
+        }
+        if (yqe >= 0 && yke == EY0) {
+            ez = ez - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
+            pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
+        }
+        if (q >= 0 && !xminus) {
+            ez = ez + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
+            pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
+        }
+        if (yqe >= 0 && yke == EYM) {
+            ez = ez - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
+            pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
+        }
+        return ez;
+    };
+
+    for (int k = 0; k < nsteps; ++k) {
+        const int n = ADJ ? n0 - k : n0 + k;
+        if (!ADJ) {
+            // receivers of this tile, sampled before the step
+            for (int e = lst + threadIdx.x; e < lend; e += NT) {
+                const int t = a.rx_idx[e];
+                const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
+                const int r = t - (t / d.nrx) * d.nrx;
+                if (li < 0 || li >= RI || lj < 0 || lj >= 64) continue;
+                d.traces[((size_t)s * d.nt + n) * d.nrx + r] = sEz[li * 64 + lj];
+            }
+            h_half();
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                if (!(act & (1u << m))) continue;
+                const int r = w + W * m, gi = gi0 + r;
+                const int o = r * 64 + lane;
+                float ez = e_cell(m, r, gi, o);
+                for (int kk = 0; kk < d.nsrc; ++kk) {
+                    const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
+                    if (sc[0] == gi && sc[1] == gj) ez = ez - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
+                }
+                sEz[o] = ez;
+                if (d.snaps && r >= K && r < K + TI && lane >= K && lane < K + TJ)
+                    d.snaps[(((size_t)n * d.S + s) * NI + gi) * pitch + gj] = ez;
+            }
+            __syncthreads();
+        } else {
+            // adjoint source at the receivers inside the region (back.cu:24-37)
+            for (int e = lst + threadIdx.x; e < lend; e += NT) {
+                const int t = a.rx_idx[e];
+                const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
+                const int r = t - (t / d.nrx) * d.nrx;
+                if (li < 0 || li >= RI || lj < 0 || lj >= 64) continue;
+                atomicAdd(&sEz[li * 64 + lj], -d.resid[((size_t)s * d.nt + n) * d.nrx + r]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                if (!(act & (1u << m))) continue;
+                const int r = w + W * m, gi = gi0 + r;
+                const int o = r * 64 + lane;
+                const float lam = e_cell(m, r, gi, o);
+                sEz[o] = lam;
+                if (r >= K && r < K + TI && lane >= K && lane < K + TJ) {
+                    const size_t so = (((size_t)n * d.S + s) * NI + gi) * pitch + gj;
+                    const float fn = (k == 0) ? d.snaps[so] : fnext[m];
+                    const float fp = d.snaps[so - (size_t)d.S * NI * pitch];
+                    ae[m] += (double)((fn - fp) * lam);
+                    as_[m] += (double)(fn * lam);
+                    fnext[m] = fp;
+                }
+            }
+            __syncthreads();
+            h_half();
+            __syncthreads();
+        }
+    }
+
+    // ---- write back the tile centre ----
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int r = w + W * m, gi = gi0 + r;
+        if (!(r >= K && r < K + TI && lane >= K && lane < K + TJ)) continue;
+        if (!(colx && gi >= 0 && gi <= NI)) continue;
+        const int o = r * 64 + lane;
+        const size_t g = shot_off + (size_t)gi * pitch + gj;
+        a.Ez_out[g] = sEz[o];
+        a.Hx_out[g] = sHx[o];
+        a.Hy_out[g] = sHy[o];
+        float* pout = a.psi_out;
+        if (xqh[m] >= 0)
+            pout[(size_t)(gi < P ? HX0 : HXM) * xslab_floats(d) + ((size_t)s * P + xqh[m]) * NJ + gj] = pHx[m];
+        if (xqe[m] >= 0)
+            pout[(size_t)(gi <= P ? EX0 : EXM) * xslab_floats(d) + ((size_t)s * P + xqe[m]) * NJ + gj] = pEx[m];
+        if ((act & (1u << m)) && yqh >= 0)
+            pout[4 * xslab_floats(d) + (size_t)(ykh - 4) * yslab_floats(d) + ((size_t)s * NI + gi) * P + yqh] = pHy[m];
+        if ((act & (1u << m)) && yqe >= 0)
+            pout[4 * xslab_floats(d) + (size_t)(yke - 4) * yslab_floats(d) + ((size_t)s * NI + gi) * P + yqe] = pEy[m];
+        if (ADJ && gi < NI && gj < NJ) {
+            double* g0 = a.gshot + ((size_t)s * NI + gi) * NJ + gj;
+            double* g1 = g0 + (size_t)d.S * NI * NJ;
+            *g0 += ae[m];
+            *g1 += as_[m];
+        }
+    }
+}
+
+// grad[c] += sum_s gshot[p][s][c] in shot order (deterministic).
+__global__ void k_reduce_shots(const double* __restrict__ gshot, int S, int64_t ncell,
+                               double* grad_eps, double* grad_sig) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    double se = 0.0, ss = 0.0;
+    for (int s = 0; s < S; ++s) {
+        se += gshot[(size_t)s * ncell + c];
+        ss += gshot[((size_t)S + s) * ncell + c];
+    }
+    grad_eps[c] += se;
+    grad_sig[c] += ss;
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers.
 dim3 cell_grid(const fgpr_problem& p) {
     return dim3((p.NJ + kBlockJ - 1) / kBlockJ, (p.NI + kBlockI - 1) / kBlockI, 1);
@@ -398,6 +710,70 @@ int launch_adjoint_step(const Dev& d, const fgpr_problem& p, int koff, int dir, 
         hipLaunchKernelGGL(k_adj_inject, dim3((ninj + 255) / 256), dim3(256), 0, st, d, koff, dir);
     hipLaunchKernelGGL((k_e_phase<false, false, true>), grid, blk, 0, st, d, koff, dir);
     hipLaunchKernelGGL(k_h_phase<false>, grid, blk, 0, st, d, koff, dir);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+// ---- time-blocked launches ----
+using TBKernel = void (*)(TBArgs, int, int, int);
+
+#define FGPR_TB_CASE(RI_, NT_, K_)                                                  \
+    if (ri == RI_ && nt == NT_ && k == K_)                                         \
+        return adj ? (TBKernel)k_tb<RI_, NT_, K_, true> : (TBKernel)k_tb<RI_, NT_, K_, false>;
+
+TBKernel tb_kernel(int ri, int nt, int k, bool adj) {
+    FGPR_TB_CASE(32, 256, 1) FGPR_TB_CASE(32, 256, 2) FGPR_TB_CASE(32, 256, 3)
+    FGPR_TB_CASE(32, 256, 4) FGPR_TB_CASE(32, 256, 6)
+    FGPR_TB_CASE(64, 512, 1) FGPR_TB_CASE(64, 512, 2) FGPR_TB_CASE(64, 512, 3)
+    FGPR_TB_CASE(64, 512, 4) FGPR_TB_CASE(64, 512, 6) FGPR_TB_CASE(64, 512, 8)
+    FGPR_TB_CASE(64, 256, 2) FGPR_TB_CASE(64, 256, 4) FGPR_TB_CASE(64, 256, 8)
+    return nullptr;
+}
+#undef FGPR_TB_CASE
+
+bool tb_enabled(const fgpr_problem& p) { return p.tb_k > 0; }
+
+int tb_check(const fgpr_problem& p, bool adj) {
+    if (!tb_kernel(p.tb_ri, p.tb_nt, p.tb_k, adj))
+        return fail(FGPR_EINVAL, "unsupported time-blocking variant (tb_ri, tb_nt, tb_k)");
+    const int TI = p.tb_ri - 2 * p.tb_k - 1, TJ = kTBCols - 2 * p.tb_k - 1;
+    if (p.tb_ti != (p.NI + 1 + TI - 1) / TI || p.tb_tj != (p.NJ + 1 + TJ - 1) / TJ)
+        return fail(FGPR_EINVAL, "tb_ti / tb_tj do not match the tile size of the variant");
+    if (!p.Ez2 || !p.Hx2 || !p.Hy2 || (p.P > 0 && !p.psi2))
+        return fail(FGPR_EINVAL, "time-blocked path needs the ping-pong buffers");
+    if (p.P > kPmax) return fail(FGPR_EINVAL, "CPML thicker than 128 cells");
+    if (adj && !p.gshot) return fail(FGPR_EINVAL, "time-blocked adjoint needs gshot");
+    if ((adj ? !p.tb_inj_ptr : !p.tb_rx_ptr) && p.nrx > 0)
+        return fail(FGPR_EINVAL, "time-blocked path needs the per-tile receiver tables");
+    return FGPR_OK;
+}
+
+TBArgs make_tb(const fgpr_problem& p, const Dev& d, int parity, bool adj) {
+    TBArgs a;
+    a.d = d;
+    const float* inE = parity ? p.Ez2 : p.Ez;
+    const float* inX = parity ? p.Hx2 : p.Hx;
+    const float* inY = parity ? p.Hy2 : p.Hy;
+    const float* inP = parity ? p.psi2 : p.psi;
+    a.Ez_in = inE; a.Hx_in = inX; a.Hy_in = inY; a.psi_in = inP;
+    a.Ez_out = parity ? p.Ez : p.Ez2;
+    a.Hx_out = parity ? p.Hx : p.Hx2;
+    a.Hy_out = parity ? p.Hy : p.Hy2;
+    a.psi_out = parity ? p.psi : p.psi2;
+    a.ti_count = p.tb_ti;
+    a.tj_count = p.tb_tj;
+    a.rx_ptr = adj ? p.tb_inj_ptr : p.tb_rx_ptr;
+    a.rx_idx = adj ? p.tb_inj_idx : p.tb_rx_idx;
+    a.gshot = p.gshot;
+    return a;
+}
+
+int launch_tb(const fgpr_problem& p, const Dev& d, bool adj, int parity, int koff, int dir,
+              int nsteps, hipStream_t st) {
+    TBKernel kfn = tb_kernel(p.tb_ri, p.tb_nt, p.tb_k, adj);
+    const TBArgs a = make_tb(p, d, parity, adj);
+    const int nwg = p.tb_ti * p.tb_tj * p.S;
+    hipLaunchKernelGGL(kfn, dim3(nwg), dim3(p.tb_nt), 0, st, a, koff, dir, nsteps);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }
@@ -441,8 +817,15 @@ int get_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
         (void)hipStreamDestroy(cs);
         return fail(FGPR_EHIP, "hipStreamBeginCapture failed");
     }
-    for (int k = 0; k < chunk && rc == FGPR_OK; ++k)
-        rc = kind == 0 ? launch_forward_step(d, p, k, dir, cs) : launch_adjoint_step(d, p, k, dir, cs);
+    if (tb_enabled(p)) {
+        // chunk is a multiple of 2*tb_k: an even number of ping-pong launches,
+        // so every replay starts from the primary buffers.
+        for (int c = 0; c * p.tb_k < chunk && rc == FGPR_OK; ++c)
+            rc = launch_tb(p, d, kind == 1, c & 1, c * p.tb_k, dir, p.tb_k, cs);
+    } else {
+        for (int k = 0; k < chunk && rc == FGPR_OK; ++k)
+            rc = kind == 0 ? launch_forward_step(d, p, k, dir, cs) : launch_adjoint_step(d, p, k, dir, cs);
+    }
     hipLaunchKernelGGL(k_advance_ctr, dim3(1), dim3(64), 0, cs, p.ctr, dir * chunk);
     hipError_t ec = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -479,6 +862,8 @@ extern "C" {
 
 int fgpr_abi_version(void) { return FGPR_ABI_VERSION; }
 
+size_t fgpr_problem_size(void) { return sizeof(fgpr_problem); }
+
 const char* fgpr_last_error(void) { return g_err.c_str(); }
 
 int fgpr_pitch(int NJ) { return ((NJ + 1 + 63) / 64) * 64; }
@@ -488,6 +873,17 @@ int64_t fgpr_psi_floats(int NI, int NJ, int P, int S) {
 }
 
 int fgpr_validate(const fgpr_problem* p) { return check_problem(p); }
+
+int fgpr_tb_tiles(int NI, int NJ, int tb_ri, int tb_nt, int tb_k, int* TI, int* TJ,
+                  int* tiles_i, int* tiles_j) {
+    if (!tb_kernel(tb_ri, tb_nt, tb_k, false)) return fail(FGPR_EINVAL, "unsupported time-blocking variant");
+    const int ti = tb_ri - 2 * tb_k - 1, tj = kTBCols - 2 * tb_k - 1;
+    if (TI) *TI = ti;
+    if (TJ) *TJ = tj;
+    if (tiles_i) *tiles_i = (NI + 1 + ti - 1) / ti;
+    if (tiles_j) *tiles_j = (NJ + 1 + tj - 1) / tj;
+    return FGPR_OK;
+}
 
 int fgpr_cell_coeffs(const float* er, const float* se, const float* mr, int NI, int NJ,
                      float dt, float dx, int adjoint, float* coef, void* stream) {
@@ -506,9 +902,12 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
     if (n0 < 0 || n1 > p->nt || n0 > n1) return fail(FGPR_EINVAL, "bad step range");
     if (!p->traces && p->nrx > 0) return fail(FGPR_EINVAL, "null traces");
     if (p->nsrc > 0 && (!p->src || !p->srcscale)) return fail(FGPR_EINVAL, "null source arrays");
+    const bool tb = tb_enabled(*p);
+    if (tb && (rc = tb_check(*p, false))) return rc;
     hipStream_t st = (hipStream_t)stream;
     fgpr_problem q = *p;
     int n = n0;
+    if (tb && chunk % (2 * p->tb_k)) chunk -= chunk % (2 * p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n1 - n0 >= chunk) {
         hipGraphExec_t ex;
         rc = get_graph(q, 0, chunk, &ex);
@@ -518,6 +917,17 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
     }
     q.ctr = nullptr;
     const Dev d = to_dev(q);
+    if (tb) {
+        int parity = 0;
+        while (n < n1) {
+            const int ns = (n1 - n < p->tb_k) ? n1 - n : p->tb_k;
+            rc = launch_tb(q, d, false, parity, n, 1, ns, st);
+            if (rc) return rc;
+            parity ^= 1;
+            n += ns;
+        }
+        return FGPR_OK;
+    }
     for (; n < n1; ++n) {
         rc = launch_forward_step(d, q, n, 1, st);
         if (rc) return rc;
@@ -533,9 +943,12 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
     if (n_lo < 1 || n_hi >= p->nt || n_hi < n_lo - 1) return fail(FGPR_EINVAL, "bad step range");
     if (!p->snaps || !p->grad_eps || !p->grad_sig) return fail(FGPR_EINVAL, "adjoint needs snaps and gradients");
     if (p->nrx > 0 && (!p->resid || !p->rx)) return fail(FGPR_EINVAL, "null residual/receivers");
+    const bool tb = tb_enabled(*p);
+    if (tb && (rc = tb_check(*p, true))) return rc;
     hipStream_t st = (hipStream_t)stream;
     fgpr_problem q = *p;
     int n = n_hi;
+    if (tb && chunk % (2 * p->tb_k)) chunk -= chunk % (2 * p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n_hi - n_lo + 1 >= chunk) {
         hipGraphExec_t ex;
         rc = get_graph(q, 1, chunk, &ex);
@@ -545,6 +958,21 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
     }
     q.ctr = nullptr;
     const Dev d = to_dev(q);
+    if (tb) {
+        int parity = 0;
+        while (n >= n_lo) {
+            const int ns = (n - n_lo + 1 < p->tb_k) ? n - n_lo + 1 : p->tb_k;
+            rc = launch_tb(q, d, true, parity, n, 1, ns, st);
+            if (rc) return rc;
+            parity ^= 1;
+            n -= ns;
+        }
+        const int64_t ncell = (int64_t)p->NI * p->NJ;
+        hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
+                           p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);
+        HIP_TRY(hipGetLastError());
+        return FGPR_OK;
+    }
     for (; n >= n_lo; --n) {
         rc = launch_adjoint_step(d, q, n, 1, st);
         if (rc) return rc;
@@ -555,6 +983,49 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
 int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* stream) {
     if (!g || !out || n < 1) return fail(FGPR_EINVAL, "bad normalise arguments");
     hipLaunchKernelGGL(k_normalise, dim3(1), dim3(1024), 0, (hipStream_t)stream, g, n, scale, out);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void* stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (n < 0 || n >= p->nt || reps < 1) return fail(FGPR_EINVAL, "bad step or repetition count");
+    fgpr_problem q = *p;
+    q.ctr = nullptr;
+    const Dev d = to_dev(q);
+    const dim3 grid = cell_grid(q), blk(kBlockJ, kBlockI, 1);
+    hipStream_t st = (hipStream_t)stream;
+    const int ninj = q.S * q.nrx;
+    for (int r = 0; r < reps; ++r) {
+        switch (kernel) {
+            case FGPR_KERNEL_FWD_H:
+                if (!q.traces && q.nrx > 0) return fail(FGPR_EINVAL, "null traces");
+                hipLaunchKernelGGL(k_h_phase<true>, grid, blk, 0, st, d, n, 1);
+                break;
+            case FGPR_KERNEL_FWD_E:
+                if (q.nsrc > 0 && (!q.src || !q.srcscale)) return fail(FGPR_EINVAL, "null source arrays");
+                if (q.snaps)
+                    hipLaunchKernelGGL((k_e_phase<true, true, false>), grid, blk, 0, st, d, n, 1);
+                else
+                    hipLaunchKernelGGL((k_e_phase<true, false, false>), grid, blk, 0, st, d, n, 1);
+                break;
+            case FGPR_KERNEL_ADJ_INJECT:
+                if (ninj > 0 && q.resid)
+                    hipLaunchKernelGGL(k_adj_inject, dim3((ninj + 255) / 256), dim3(256), 0, st, d, n, 1);
+                break;
+            case FGPR_KERNEL_ADJ_E:
+                if (n < 1 || !q.snaps || !q.grad_eps || !q.grad_sig)
+                    return fail(FGPR_EINVAL, "adjoint E kernel needs n >= 1, snaps and gradients");
+                hipLaunchKernelGGL((k_e_phase<false, false, true>), grid, blk, 0, st, d, n, 1);
+                break;
+            case FGPR_KERNEL_ADJ_H:
+                hipLaunchKernelGGL(k_h_phase<false>, grid, blk, 0, st, d, n, 1);
+                break;
+            default:
+                return fail(FGPR_EINVAL, "unknown kernel id");
+        }
+    }
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

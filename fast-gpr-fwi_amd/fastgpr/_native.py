@@ -19,10 +19,11 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
+KERNEL_FWD_H, KERNEL_FWD_E, KERNEL_ADJ_INJECT, KERNEL_ADJ_E, KERNEL_ADJ_H = range(5)
 
 
 class Problem(ctypes.Structure):
@@ -39,6 +40,13 @@ class Problem(ctypes.Structure):
         ("psi", ctypes.c_void_p), ("traces", ctypes.c_void_p), ("snaps", ctypes.c_void_p),
         ("resid", ctypes.c_void_p), ("grad_eps", ctypes.c_void_p), ("grad_sig", ctypes.c_void_p),
         ("ctr", ctypes.c_void_p),
+        ("Ez2", ctypes.c_void_p), ("Hx2", ctypes.c_void_p), ("Hy2", ctypes.c_void_p),
+        ("psi2", ctypes.c_void_p),
+        ("tb_ri", ctypes.c_int32), ("tb_nt", ctypes.c_int32), ("tb_k", ctypes.c_int32),
+        ("tb_ti", ctypes.c_int32), ("tb_tj", ctypes.c_int32),
+        ("tb_rx_ptr", ctypes.c_void_p), ("tb_rx_idx", ctypes.c_void_p),
+        ("tb_inj_ptr", ctypes.c_void_p), ("tb_inj_idx", ctypes.c_void_p),
+        ("gshot", ctypes.c_void_p),
     ]
 
 
@@ -46,9 +54,11 @@ class Problem(ctypes.Structure):
 SIGNATURES = {
     "fgpr_abi_version": (ctypes.c_int, []),
     "fgpr_last_error": (ctypes.c_char_p, []),
+    "fgpr_problem_size": (ctypes.c_size_t, []),
     "fgpr_pitch": (ctypes.c_int, [ctypes.c_int]),
     "fgpr_psi_floats": (ctypes.c_int64, [ctypes.c_int] * 4),
     "fgpr_validate": (ctypes.c_int, [ctypes.POINTER(Problem)]),
+    "fgpr_tb_tiles": (ctypes.c_int, [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)] * 4),
     "fgpr_cell_coeffs": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int,
                                          ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p]),
@@ -58,6 +68,8 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "fgpr_normalise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "fgpr_profile_kernel": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
 }
 
@@ -75,6 +87,8 @@ def _load():
         fn.argtypes = args
     if lib.fgpr_abi_version() != ABI_VERSION:
         raise ImportError(f"libfastgpr ABI {lib.fgpr_abi_version()} != expected {ABI_VERSION}")
+    if lib.fgpr_problem_size() != ctypes.sizeof(Problem):
+        raise ImportError("fgpr_problem layout mismatch between libfastgpr and the ctypes mirror")
     return lib
 
 
@@ -87,6 +101,13 @@ def check(rc: int, what: str = "") -> None:
     if rc != 0:
         msg = lib.fgpr_last_error().decode(errors="replace")
         raise RuntimeError(f"fastgpr{(' ' + what) if what else ''} failed ({rc}): {msg}")
+
+
+def tb_tiles(NI: int, NJ: int, ri: int, nt: int, k: int):
+    """(TI, TJ, tiles_i, tiles_j) of a time-blocked variant."""
+    out = [ctypes.c_int() for _ in range(4)]
+    check(lib.fgpr_tb_tiles(NI, NJ, ri, nt, k, *[ctypes.byref(o) for o in out]), "tb_tiles")
+    return tuple(o.value for o in out)
 
 
 def pitch(NJ: int) -> int:

@@ -11,6 +11,9 @@ Extra keyword arguments (all default to the reference behaviour):
 * ``exec_mode``   -- "graph" (time loop replayed from captured HIP graphs) or
                      "eager" (one kernel sequence per step from the host loop).
 * ``graph_chunk`` -- steps per captured graph.
+* ``time_block``  -- None (one H and one E kernel per step), "auto" (the
+                     autotuner picks a time-blocked variant for this grid) or
+                     an explicit (region rows, threads, steps per launch).
 """
 from __future__ import annotations
 
@@ -50,7 +53,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             receiver_location=None, receiver_step=None, pmlthick=0, freq=None,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
-            exec_mode="graph", graph_chunk=32):
+            exec_mode="graph", graph_chunk=32, time_block=None):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
     Same arguments and meaning as compute.py:33-50: ``nx`` is the depth
@@ -64,7 +67,10 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
                        source_step, receiver_location, receiver_step, pmlthick,
                        source_apmlitudes, step, er, se, mr, nsrc, nrx, dt, pml_formulation,
                        freq, total_step, tv, Fixedreceiver, nseg, customsource, customreceiver,
-                       exec_mode=exec_mode, graph_chunk=graph_chunk)
+                       exec_mode=exec_mode, graph_chunk=graph_chunk, time_block=time_block)
+    if G.time_block == "auto":
+        from .autotune import choose
+        G.time_block = choose(G, er, se, mr)
     rxs = GPRFWI.apply(G, er, se, mr)
     return rxs, G.dt
 
@@ -73,7 +79,7 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                    receiver_location, receiver_step, pmlthick, source_apmlitudes, step, er, se,
                    mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
                    nseg, customsource=None, customreceiver=None, exec_mode="graph",
-                   graph_chunk=32) -> FDTDGrid:
+                   graph_chunk=32, time_block=None) -> FDTDGrid:
     """Build the solve description (compute.py:726-872).
 
     ``nseg`` (the reference's unfinished segmenting, compute.py:543-553) and
@@ -113,7 +119,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
     return FDTDGrid(device=device, NI=NI, NJ=NJ, S=S, nsrc=int(src.shape[1]), nrx=int(rx.shape[1]),
                     dx=dx, dt=Gdt, iterations=nt, pmlthick=P, source=src, receiver=rx,
                     srcwaves=srcwaves, pml=pml, tv=tv, exec_mode=exec_mode,
-                    graph_chunk=int(graph_chunk))
+                    graph_chunk=int(graph_chunk),
+                    time_block=time_block if time_block in (None, "auto") else tuple(time_block))
 
 
 class GPRFWI(torch.autograd.Function):

@@ -185,6 +185,7 @@ class FDTDGrid:
     tv: bool = True
     exec_mode: str = "graph"
     graph_chunk: int = 32
+    time_block: tuple | None = None   # (region rows, threads, steps per launch) or None
     options: dict = field(default_factory=dict)
 
     @property

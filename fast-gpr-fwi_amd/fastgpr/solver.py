@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _native
@@ -48,10 +49,54 @@ def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device) -> 
     return coef
 
 
-class Batch:
-    """Device buffers of one batch of S shots on one grid."""
+def _csr(keys: np.ndarray, vals: np.ndarray, nkeys: int):
+    order = np.argsort(keys, kind="stable")
+    ptr = np.zeros(nkeys + 1, np.int32)
+    np.add.at(ptr, keys + 1, 1)
+    return np.cumsum(ptr).astype(np.int32), vals[order].astype(np.int32)
 
-    def __init__(self, G, store: bool = True):
+
+def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
+    """Receiver tables of the time-blocked kernels (host, numpy).
+
+    rx: (S, nrx, 2) int cell indices, i < 0 = dead.  Forward: each live
+    receiver is sampled by the one tile whose centre holds it.  Adjoint: each
+    receiver on an updated cell (i < NI, j < NJ) is injected by every tile
+    whose region (centre + halo) holds it."""
+    ri, nt_, k = tb
+    TI, TJ, ti, tj = _native.tb_tiles(NI, NJ, ri, nt_, k)
+    S, nrx = rx.shape[0], rx.shape[1]
+    ntiles = ti * tj
+    fk, fv, ak, av = [], [], [], []
+    for s in range(S):
+        for r in range(nrx):
+            i, j = int(rx[s, r, 0]), int(rx[s, r, 1])
+            if i < 0:
+                continue
+            ent = s * nrx + r
+            fk.append(s * ntiles + (i // TI) * tj + (j // TJ))
+            fv.append(ent)
+            if i >= NI or j >= NJ:
+                continue
+            for a in range(max(0, (i + k - ri) // TI), min(ti - 1, (i + k) // TI) + 1):
+                if not (a * TI - k <= i < a * TI - k + ri):
+                    continue
+                for b in range(max(0, (j + k - 64) // TJ), min(tj - 1, (j + k) // TJ) + 1):
+                    if b * TJ - k <= j < b * TJ - k + 64:
+                        ak.append(s * ntiles + a * tj + b)
+                        av.append(ent)
+    fptr, fidx = _csr(np.asarray(fk, np.int64), np.asarray(fv, np.int64), S * ntiles)
+    aptr, aidx = _csr(np.asarray(ak, np.int64), np.asarray(av, np.int64), S * ntiles)
+    return dict(TI=TI, TJ=TJ, ti=ti, tj=tj, fptr=fptr, fidx=fidx, aptr=aptr, aidx=aidx)
+
+
+class Batch:
+    """Device buffers of one batch of S shots on one grid.
+
+    ``tb`` = (region rows, threads, steps per launch) selects the
+    time-blocked kernels; None selects the per-step kernels."""
+
+    def __init__(self, G, store: bool = True, tb=None):
         dev = G.device
         self.G = G
         self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, G.S, G.iterations
@@ -73,10 +118,32 @@ class Batch:
         self.traces = torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev)
         self.snaps = (torch.empty((self.nt, self.S, self.NI, self.pitch), dtype=torch.float32, device=dev)
                       if store else None)
+        self.tb = tuple(tb) if tb else None
+        self.gshot = None
+        if self.tb:
+            self.fields2 = torch.zeros_like(self.fields)
+            self.psi2 = torch.zeros_like(self.psi)
+            t = tile_tables(self.rx.cpu().numpy(), self.NI, self.NJ, self.tb)
+            self.tiles = (t["ti"], t["tj"])
+            as_dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+            self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
+                              as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
 
     def reset_state(self):
         self.fields.zero_()
         self.psi.zero_()
+        if self.tb:
+            self.fields2.zero_()
+            self.psi2.zero_()
+
+    def gshot_buffer(self):
+        """Per-shot FP64 partial gradients of the time-blocked adjoint (zeroed)."""
+        if self.gshot is None:
+            self.gshot = torch.zeros((2, self.S, self.NI, self.NJ), dtype=torch.float64,
+                                     device=self.fields.device)
+        else:
+            self.gshot.zero_()
+        return self.gshot
 
     def problem(self, coef, srcscale=None, resid=None, grad_eps=None, grad_sig=None):
         G = self.G
@@ -99,6 +166,15 @@ class Batch:
         p.grad_eps = grad_eps.data_ptr() if grad_eps is not None else None
         p.grad_sig = grad_sig.data_ptr() if grad_sig is not None else None
         p.ctr = self.ctr.data_ptr()
+        if self.tb:
+            p.Ez2 = self.fields2[0].data_ptr()
+            p.Hx2 = self.fields2[1].data_ptr()
+            p.Hy2 = self.fields2[2].data_ptr()
+            p.psi2 = self.psi2.data_ptr()
+            p.tb_ri, p.tb_nt, p.tb_k = self.tb
+            p.tb_ti, p.tb_tj = self.tiles
+            p.tb_rx_ptr, p.tb_rx_idx, p.tb_inj_ptr, p.tb_inj_idx = [t.data_ptr() for t in self.tb_tables]
+            p.gshot = self.gshot.data_ptr() if self.gshot is not None else None
         return p
 
 
@@ -115,8 +191,17 @@ def forward(G, er, se, mr, store: bool = True):
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
     coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
-    b = Batch(G, store=store)
-    scale = source_scale(G).to(dev)
+    b = Batch(G, store=store, tb=G.time_block)
+    scale = source_scale(G)
+    # Rows whose waveform is identically zero inject exactly nothing
+    # (Ez - uE4*0 == Ez); only emitting rows go to the device.
+    emit = (scale != 0).any(dim=1)
+    if not bool(emit.all()):
+        b.src = b.src[:, emit.to(b.src.device)].contiguous()
+        b.nsrc = int(b.src.shape[1])
+        scale = scale[emit]
+    scale = scale.contiguous().to(dev)
+    b.srcscale = scale
     p = b.problem(coef, srcscale=scale)
     check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], G.graph_chunk,
                            _stream(dev)), "forward")
@@ -138,6 +223,8 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     NI, NJ = b.NI, b.NJ
     ge = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
     gs = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    if b.tb:
+        b.gshot_buffer()
     p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
     if b.nt >= 2:
         check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 1, _native.GRAD_REFERENCE,

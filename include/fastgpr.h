@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 1
+#define FGPR_ABI_VERSION 2
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -100,10 +100,25 @@ typedef struct fgpr_problem {
     double* grad_eps;
     double* grad_sig;
     int32_t* ctr;
+    /* Time-blocked path, used when tb_k > 0 (see fgpr_tb_tiles). */
+    float* Ez2;                 /* ping-pong partners of Ez, Hx, Hy, psi (same layouts) */
+    float* Hx2;
+    float* Hy2;
+    float* psi2;
+    int32_t tb_ri, tb_nt, tb_k; /* region rows, threads per workgroup, steps per launch */
+    int32_t tb_ti, tb_tj;       /* tiles along i and j */
+    const int32_t* tb_rx_ptr;   /* forward: receivers each (shot, tile) samples, CSR over S*tiles */
+    const int32_t* tb_rx_idx;   /*   entries s*nrx + r */
+    const int32_t* tb_inj_ptr;  /* adjoint: receivers inside each (shot, tile) region */
+    const int32_t* tb_inj_idx;
+    double* gshot;              /* adjoint: 2*S*NI*NJ per-shot FP64 partial gradients (zeroed by caller) */
 } fgpr_problem;
 
 /* ABI version (FGPR_ABI_VERSION). */
 int fgpr_abi_version(void);
+
+/* sizeof(fgpr_problem), for binding layers to check their mirror of it. */
+size_t fgpr_problem_size(void);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* fgpr_last_error(void);
@@ -113,6 +128,15 @@ int fgpr_pitch(int NJ);
 
 /* Number of floats of the psi workspace for one batch. */
 int64_t fgpr_psi_floats(int NI, int NJ, int P, int S);
+
+/*
+ * Tile geometry of a time-blocked variant: centre tile size (*TI x *TJ) and
+ * tile counts for an NI x NJ grid.  Returns FGPR_EINVAL for a variant that
+ * is not compiled in.  Supported (tb_ri, tb_nt): (32, 256), (64, 512),
+ * (64, 256); tb_k up to 8.
+ */
+int fgpr_tb_tiles(int NI, int NJ, int tb_ri, int tb_nt, int tb_k, int* TI, int* TJ,
+                  int* tiles_i, int* tiles_j);
 
 /* Check a descriptor without launching anything (0 = consistent). */
 int fgpr_validate(const fgpr_problem* p);
@@ -147,6 +171,21 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
  * Uses one deterministic single-workgroup reduction.
  */
 int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* stream);
+
+/* Kernel ids for fgpr_profile_kernel. */
+#define FGPR_KERNEL_FWD_H 0      /* H + H-CPML + receiver sampling */
+#define FGPR_KERNEL_FWD_E 1      /* E + E-CPML + source + snapshot store */
+#define FGPR_KERNEL_ADJ_INJECT 2 /* adjoint source at the receivers */
+#define FGPR_KERNEL_ADJ_E 3      /* E + E-CPML + gradient accumulation */
+#define FGPR_KERNEL_ADJ_H 4      /* H + H-CPML (adjoint coefficients) */
+
+/*
+ * Launch one solver kernel `reps` times back to back at step n, on the
+ * descriptor's buffers, for timing with stream events.  It advances the
+ * state like the real step does, so run it on scratch buffers.  No
+ * counterpart in the reference (measurement only).
+ */
+int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void* stream);
 
 /* Release every HIP graph the library captured (also done at unload). */
 int fgpr_release_graphs(void);
