@@ -36,31 +36,44 @@ def parse():
     ap.add_argument("--exec", dest="exec_mode", default="graph", choices=["graph", "eager"])
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=60,
+    ap.add_argument("--time-block", default="auto",
+                    help="'auto' (autotuned), 'none' (per-step kernels) or rows,threads,steps")
+    ap.add_argument("--plain", action="store_true",
+                    help="also time the plain one-shot-per-launch eager path")
+    ap.add_argument("--cpu-steps", type=int, default=400,
                     help="time steps of the CPU oracle sample (forward + adjoint)")
     ap.add_argument("--profile-reps", type=int, default=200)
     return ap.parse_args()
 
 
-def kernel_bytes(kernel: int, S: int, NI: int, NJ: int, P: int, nrx: int) -> float:
-    """Algorithmic HBM bytes of one launch of each solver kernel.
+def step_bytes(S: int, NI: int, NJ: int, P: int) -> float:
+    """Algorithmic HBM bytes of one time step of the batch (SURVEY 8(d) unit
+    figures): 40 B per cell-update (Ez/Hx/Hy read+write 24 B, three
+    coefficients 12 B, snapshot write or read 4 B) plus 16 B per CPML cell
+    and active direction, Npsi = 2P*NJ + 2P*NI (corners counted twice)."""
+    return S * (40.0 * NI * NJ + 16.0 * (2 * P * NJ + 2 * P * NI))
 
-    Unit figures (bytes per shot-cell unless noted): a field read or write is
-    4 B; a coefficient plane is 4 B per model cell, read once per launch and
-    shared by the S shots; the snapshot is 4 B; the CPML memory variable of a
-    slab cell is read and written (8 B) in the half-step that owns it; the
-    FP64 gradient pair is read and written once per model cell (32 B)."""
+
+def kernel_bytes(kernel: int, S: int, NI: int, NJ: int, P: int, nrx: int, k: int = 1) -> float:
+    """Algorithmic bytes of one launch of each solver kernel.
+
+    Time-blocked launches advance k steps: k x step_bytes.  The per-step
+    kernels split one step in two half-steps; each is charged the fields it
+    must read and write (4 B per shot-cell per field access), its coefficient
+    planes (4 B per model cell, shared by the S shots), the snapshot (4 B),
+    its CPML memory variables (8 B per slab cell) and, for the adjoint E
+    kernel, the FP64 gradient pair (32 B per model cell)."""
     cells = S * NI * NJ
     model = NI * NJ
     psi_cells = S * (2 * P * NJ + 2 * P * NI)
-    if kernel in (0, 4):     # H: read Ez, Hx, Hy; write Hx, Hy; uH1, uH4; H psi
+    if kernel in (5, 6):
+        return k * step_bytes(S, NI, NJ, P)
+    if kernel in (0, 4):
         b = 20 * cells + 8 * model + 8 * psi_cells
-        if kernel == 0:
-            b += 8 * S * nrx
-        return b
-    if kernel == 1:          # forward E: read Ez, Hx, Hy; write Ez, snapshot; uE0, uE1, uE4; E psi
+        return b + (8 * S * nrx if kernel == 0 else 0)
+    if kernel == 1:
         return 20 * cells + 12 * model + 8 * psi_cells
-    if kernel == 3:          # adjoint E: + two snapshot reads, FP64 gradient pair
+    if kernel == 3:
         return 24 * cells + 12 * model + 8 * psi_cells + 32 * model
     raise ValueError(kernel)
 
@@ -88,9 +101,11 @@ def main():
     er0, se0 = t(wl.er0), t(wl.se0)
     src = torch.tensor(wl.src)
     rx = torch.tensor(wl.rx)
+    tb = (None if args.time_block == "none" else
+          "auto" if args.time_block == "auto" else tuple(int(v) for v in args.time_block.split(",")))
     kw = dict(time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P, tv=False,
               customsource=src, customreceiver=rx, exec_mode=args.exec_mode,
-              graph_chunk=args.chunk)
+              graph_chunk=args.chunk, time_block=tb)
     common = (dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq], wl.S)
 
     with torch.no_grad():
@@ -121,46 +136,19 @@ def main():
     ms = 1e3 * elapsed / args.steps
     value = cu / (elapsed / args.steps) / 1e9
 
-    # ---- dominant-kernel roofline: per-kernel launch durations with HIP events
-    Gx = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
-                                None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
-                                er.detach(), se.detach(), mr, 1, wl.rx.shape[1], wl.dt,
-                                'HORIPML', None, None, False, False, None, src, rx)
-    b = S_.Batch(Gx, store=True)
-    coef = S_.cell_coeffs(er.detach(), se.detach(), mr, Gx.dt32, Gx.dx32, False, dev)
-    scale = S_.source_scale(Gx).to(dev)
-    ge = torch.zeros((wl.NI, wl.NJ), dtype=torch.float64, device=dev)
-    gs = torch.zeros_like(ge)
-    resid = torch.zeros((wl.S, b.nt, b.nrx), dtype=torch.float32, device=dev)
-    p = b.problem(coef, srcscale=scale, resid=resid, grad_eps=ge, grad_sig=gs)
-    b.snaps.zero_()
-    stream = torch.cuda.current_stream(dev)
-    launches_per_grad = {0: b.nt, 1: b.nt, 3: b.nt - 1, 4: b.nt - 1}
-    per_kernel = {}
-    import ctypes
-    for kid in (0, 1, 3, 4):
-        reps = args.profile_reps
-        _native.check(_native.lib.fgpr_profile_kernel(ctypes.byref(p), kid, 10, 5,
-                                                       ctypes.c_void_p(stream.cuda_stream)))
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        _native.check(_native.lib.fgpr_profile_kernel(ctypes.byref(p), kid, 10, reps,
-                                                       ctypes.c_void_p(stream.cuda_stream)))
-        e1.record(stream)
-        e1.synchronize()
-        dur_us = e0.elapsed_time(e1) * 1e3 / reps
-        byts = kernel_bytes(kid, wl.S, wl.NI, wl.NJ, wl.P, b.nrx)
-        per_kernel[kid] = dict(us=dur_us, bytes=byts, gbs=byts / dur_us / 1e3,
-                               share_us=dur_us * launches_per_grad[kid])
-    names = {0: "k_h_phase<true> (forward H)", 1: "k_e_phase<true,true,false> (forward E)",
-             3: "k_e_phase<false,false,true> (adjoint E + gradient)", 4: "k_h_phase<false> (adjoint H)"}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["share_us"])
-    d = per_kernel[dom]
-    roofline = {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": names[dom], "avg_launch_us": round(d["us"], 2),
-                "bytes_per_launch": int(d["bytes"])}
+    # ---- dominant-kernel roofline: launch durations with HIP events on the
+    # launch stream, on scratch buffers (same shapes as the timed run)
+    if tb == "auto":
+        from fastgpr.autotune import _cache, _key
+        Gk = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                                    None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                                    er.detach(), se.detach(), mr, 1, wl.rx.shape[1], wl.dt,
+                                    'HORIPML', None, None, False, False, None, src, rx)
+        tb_used = _cache[_key(Gk)]
+    else:
+        tb_used = tb
+    roofline, per_kernel = profile_kernels(wl, er.detach(), se.detach(), mr, src, rx, tb_used,
+                                           args.profile_reps)
 
     # ---- CPU baseline: the numpy FP64 oracle on a bounded sample
     cpu = None
@@ -183,12 +171,112 @@ def main():
         "config": {"workload": wl.name, "grid": [wl.NI, wl.NJ], "dx_m": wl.dx, "cpml": wl.P,
                    "nt": wl.nt, "shots": wl.S, "receivers": int(wl.rx.shape[1]),
                    "cell_updates_per_gradient": cu, "exec": args.exec_mode,
-                   "graph_chunk": args.chunk},
+                   "graph_chunk": args.chunk,
+                   "time_block": list(tb_used) if tb_used else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels_us": {names[k]: round(v["us"], 2) for k, v in per_kernel.items()},
+        "kernels_us": {k: round(v["us"], 2) for k, v in per_kernel.items()},
     }
+    if args.plain:
+        out["plain_path"] = plain_path(wl, er.detach(), se.detach(), mr, obs, src, rx, cu)
+        out["plain_path"]["speedup"] = round(value / out["plain_path"]["value"], 2)
     print(json.dumps(out), flush=True)
+
+
+KERNEL_NAMES = {
+    0: "k_h_phase<true> (forward H half-step)",
+    1: "k_e_phase<true,true,false> (forward E half-step)",
+    3: "k_e_phase<false,false,true> (adjoint E half-step + gradient)",
+    4: "k_h_phase<false> (adjoint H half-step)",
+    5: "k_tb<{ri},{nt},{k},false> (time-blocked forward, {k} steps)",
+    6: "k_tb<{ri},{nt},{k},true> (time-blocked adjoint + gradient, {k} steps)",
+}
+
+
+def profile_kernels(wl, er, se, mr, src, rx, tb, reps):
+    """Average launch duration of each kernel of the path (HIP events on the
+    stream the kernels run on), its algorithmic bytes and GB/s; the dominant
+    kernel is the one with the largest share of a gradient's time."""
+    import ctypes
+    import torch
+    import fastgpr
+    from fastgpr import _native
+    from fastgpr import solver as S_
+    dev = er.device
+    G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                               None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                               er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
+                               False, False, None, src, rx, time_block=tb)
+    b = S_.Batch(G, store=True, tb=tb)
+    coef = S_.cell_coeffs(er, se, mr, G.dt32, G.dx32, False, dev)
+    scale = S_.source_scale(G).to(dev)
+    ge = torch.zeros((wl.NI, wl.NJ), dtype=torch.float64, device=dev)
+    gs = torch.zeros_like(ge)
+    resid = torch.zeros((wl.S, b.nt, b.nrx), dtype=torch.float32, device=dev)
+    if tb:
+        b.gshot_buffer()
+    p = b.problem(coef, srcscale=scale, resid=resid, grad_eps=ge, grad_sig=gs)
+    b.snaps.zero_()
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    if tb:
+        k = tb[2]
+        kids = {5: (100, b.nt / k), 6: (100 + k, b.nt / k)}
+    else:
+        k = 1
+        kids = {0: (10, b.nt), 1: (10, b.nt), 3: (10, b.nt - 1), 4: (10, b.nt - 1)}
+    per = {}
+    best = None
+    for kid, (n, launches) in kids.items():
+        _native.check(_native.lib.fgpr_profile_kernel(ctypes.byref(p), kid, n, 4, sp))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _native.check(_native.lib.fgpr_profile_kernel(ctypes.byref(p), kid, n, reps, sp))
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        byts = kernel_bytes(kid, wl.S, wl.NI, wl.NJ, wl.P, b.nrx, k)
+        name = KERNEL_NAMES[kid].format(ri=tb[0] if tb else 0, nt=tb[1] if tb else 0, k=k)
+        per[name] = dict(us=us, bytes=byts, gbs=byts / us / 1e3, share=us * launches)
+        if best is None or per[name]["share"] > per[best]["share"]:
+            best = name
+    d = per[best]
+    roofline = {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": best, "avg_launch_us": round(d["us"], 2),
+                "bytes_per_launch": int(d["bytes"])}
+    return roofline, per
+
+
+def plain_path(wl, er, se, mr, obs, src, rx, cu):
+    """The plain GPU path: one shot per launch sequence, per-step kernels
+    launched eagerly from the host loop, no graph, no batching, no time
+    blocking.  One full gradient of the same workload, shot by shot."""
+    import torch
+    import fastgpr
+    dev = er.device
+    er = er.clone().requires_grad_(True)
+    se = se.clone().requires_grad_(True)
+    common = (dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq], 1)
+
+    def one_gradient():
+        for s in range(wl.S):
+            pred, _ = fastgpr.compute(*common, er, se, mr, 1, wl.rx.shape[1],
+                                      time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P,
+                                      tv=False, customsource=src[s:s + 1],
+                                      customreceiver=rx[s:s + 1], exec_mode="eager")
+            (0.5 * ((pred - obs[s:s + 1]) ** 2).sum()).backward()
+
+    one_gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    one_gradient()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"value": round(cu / el / 1e9, 3), "unit": "Gcell-updates/s",
+            "ms_per_gradient": round(el * 1e3, 1),
+            "what": "per-step eager launches, one shot per launch sequence, no graph"}
 
 
 def cpu_baseline(wl, nsteps: int):
@@ -201,9 +289,10 @@ def cpu_baseline(wl, nsteps: int):
     pb = oracle_problem(wl)
     pb.nt = nsteps
     pb.wavelet = pb.wavelet[:, :nsteps]
+    resid = np.random.default_rng(0).standard_normal((wl.S, nsteps, wl.rx.shape[1]))
     t0 = time.perf_counter()
     tr, snaps = O.run_forward(pb, np.float64, store=True)
-    O.run_gradient(pb, tr, snaps, np.float64)
+    O.adjoint_reference(*O.setup(pb, np.float64, adjoint=True), pb.P, pb.dx, pb.rx, resid, snaps)
     el = time.perf_counter() - t0
     cu = 2 * wl.S * wl.NI * wl.NJ * nsteps
     cores = 1

@@ -61,6 +61,7 @@ enum Slab { EX0 = 0, EXM = 1, HX0 = 2, HXM = 3, EY0 = 4, EYM = 5, HY0 = 6, HYM =
 
 struct Dev {
     int NI, NJ, P, S, nt, nsrc, nrx, pitch;
+    int snap_base, snap_mod;
     float dx;
     const float* __restrict__ coef;
     const float* __restrict__ pml;
@@ -83,6 +84,8 @@ Dev to_dev(const fgpr_problem& p) {
     Dev d;
     d.NI = p.NI; d.NJ = p.NJ; d.P = p.P; d.S = p.S; d.nt = p.nt;
     d.nsrc = p.nsrc; d.nrx = p.nrx; d.pitch = p.pitch; d.dx = p.dx;
+    d.snap_base = p.snap_base;
+    d.snap_mod = p.snap_mod;
     d.coef = p.coef; d.pml = p.pml; d.src = p.src; d.srcscale = p.srcscale; d.rx = p.rx;
     d.Ez = p.Ez; d.Hx = p.Hx; d.Hy = p.Hy; d.psi = p.psi;
     d.traces = p.traces; d.snaps = p.snaps; d.resid = p.resid;
@@ -91,6 +94,13 @@ Dev to_dev(const fgpr_problem& p) {
 }
 
 __device__ __forceinline__ size_t plane(const Dev& d) { return (size_t)(d.NI + 1) * d.pitch; }
+
+// Offset of snapshot (step n, shot s, cell i, j): slot n - snap_base, or
+// n mod snap_mod for a ring of slots (checkpointed segments).
+__device__ __forceinline__ size_t snap_off(const Dev& d, int n, int s, int i, int j) {
+    const int slot = d.snap_mod ? n % d.snap_mod : n - d.snap_base;
+    return (((size_t)slot * d.S + s) * d.NI + i) * d.pitch + j;
+}
 
 __device__ __forceinline__ size_t xslab_floats(const Dev& d) { return (size_t)d.S * d.P * d.NJ; }
 __device__ __forceinline__ size_t yslab_floats(const Dev& d) { return (size_t)d.S * d.NI * d.P; }
@@ -319,10 +329,10 @@ k_e_phase(Dev d, int koff, int dir) {
             }
         }
         d.Ez[b] = ez;
-        if (SNAP) d.snaps[(((size_t)n * d.S + s) * NI + i) * pitch + j] = ez;
+        if (SNAP) d.snaps[snap_off(d, n, s, i, j)] = ez;
         if (GRAD) {
-            const size_t sn = (((size_t)n * d.S + s) * NI + i) * pitch + j;
-            const size_t sp = sn - (size_t)d.S * NI * pitch;
+            const size_t sn = snap_off(d, n, s, i, j);
+            const size_t sp = snap_off(d, n - 1, s, i, j);
             const float fn = d.snaps[sn];
             const float fp = d.snaps[sp];
             acc_e += (double)((fn - fp) * ez);
@@ -350,6 +360,163 @@ __global__ void k_adj_inject(Dev d, int koff, int dir) {
     atomicAdd(&d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj], -g);
 }
 
+// ---------------------------------------------------------------------------
+// Exact discrete adjoint: the transpose of the forward time stepping, for
+// gradients that finite differences of the forward map can check.  Reverse
+// step n takes the adjoints of (E^{n+1}, H^{n+1/2}, psi) to those of
+// (E^n, H^{n-1/2}, psi) in four launches:
+//   A: per E cell -- gradient terms, the curl coefficient times lambda (U),
+//      the E-CPML transposes (Lx, Ly) and the psi_E adjoints;
+//   B: per H cell -- gather U / Lx / Ly into lambda_H, H-CPML transposes
+//      (Mx, My, psi_H adjoints) and the H curl terms (Vx, Vy);
+//   C: per E cell -- lambda_E = (uE0 or 1) lambda + gathers of V / M;
+//   D: lambda_E at the receivers += dL/dtraces[n].
+// dE^{n+1}/d er = -(e0/dt) uE4 (E^{n+1} - E^n) and, on cells the main update
+// covers, dE^{n+1}/d se = -uE4 (E^{n+1} + E^n)/2 (every other term of the E
+// update is uE4 or uE1 = uE4/dx times something independent of the cell's
+// own er / se).  grad_eps accumulates sum uE4*lambda*(E^{n+1}-E^n); the
+// constant -(e0/dt) is applied by the caller.
+struct XScratch {
+    float* U; float* Lx; float* Ly; float* Vx; float* Vy; float* Mx; float* My;
+};
+
+__device__ __forceinline__ XScratch xscratch(const Dev& d, float* base) {
+    const size_t f = (size_t)d.S * plane(d);
+    XScratch x;
+    x.U = base; x.Lx = base + f; x.Ly = base + 2 * f; x.Vx = base + 3 * f;
+    x.Vy = base + 4 * f; x.Mx = base + 5 * f; x.My = base + 6 * f;
+    return x;
+}
+
+__global__ void __launch_bounds__(kBlockJ* kBlockI)
+k_xadj_a(Dev d, int koff, int dir, float* scratch) {
+    const int n = step_index(d, koff, dir);
+    const int j = blockIdx.x * kBlockJ + threadIdx.x;
+    const int i = blockIdx.y * kBlockI + threadIdx.y;
+    const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
+    if (i >= NI || j >= NJ) return;
+    const XScratch x = xscratch(d, scratch);
+    const size_t pl = plane(d);
+    const size_t o = (size_t)i * pitch + j;
+    const float uE1 = d.coef[pl + o];
+    const float uE4 = d.coef[2 * pl + o];
+    const bool mainE = (i >= 1 && j >= 1);
+    int xq = -1, xk = 0, yq = -1, yk = 0;
+    if (P > 0) {
+        if (i >= 1 && i <= P) { xq = P - i; xk = EX0; }
+        else if (i >= NI - P) { xq = i - (NI - P); xk = EXM; }
+        if (j >= 1 && j <= P) { yq = P - j; yk = EY0; }
+        else if (j >= NJ - P) { yq = j - (NJ - P); yk = EYM; }
+    }
+    double acc_e = 0.0, acc_s = 0.0;
+    for (int s = 0; s < d.S; ++s) {
+        const size_t b = (size_t)s * pl + o;
+        const float e = d.Ez[b];
+        const double f1 = d.snaps[snap_off(d, n, s, i, j)];
+        const double f0 = n >= 1 ? (double)d.snaps[snap_off(d, n - 1, s, i, j)] : 0.0;
+        const double ue = (double)uE4 * (double)e;
+        acc_e += ue * (f1 - f0);
+        acc_s += mainE ? -0.5 * ue * (f1 + f0) : -0.5 * ue * (f1 - f0);
+        x.U[b] = mainE ? uE1 * e : 0.f;
+        float lx = 0.f, ly = 0.f;
+        if (xq >= 0) {
+            float* pm = psi_x(d, xk, s, xq, j);
+            const float m = *pm;
+            lx = uE4 * (d.pml[xq] - 1.0f) * e - d.pml[3 * P + xq] * m;
+            *pm = uE4 * d.pml[P + xq] * e + d.pml[2 * P + xq] * m;
+        }
+        if (yq >= 0) {
+            float* pm = psi_y(d, yk, s, i, yq);
+            const float m = *pm;
+            ly = -uE4 * (d.pml[yq] - 1.0f) * e - d.pml[3 * P + yq] * m;
+            *pm = -uE4 * d.pml[P + yq] * e + d.pml[2 * P + yq] * m;
+        }
+        x.Lx[b] = lx;
+        x.Ly[b] = ly;
+    }
+    const size_t g = (size_t)i * NJ + j;
+    d.grad_eps[g] += acc_e;
+    d.grad_sig[g] += acc_s;
+}
+
+__global__ void __launch_bounds__(kBlockJ* kBlockI)
+k_xadj_b(Dev d, float* scratch) {
+    const int j = blockIdx.x * kBlockJ + threadIdx.x;
+    const int i = blockIdx.y * kBlockI + threadIdx.y;
+    const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
+    if (i >= NI || j >= NJ) return;
+    const XScratch x = xscratch(d, scratch);
+    const size_t pl = plane(d);
+    const size_t o = (size_t)i * pitch + j;
+    const float uH1 = d.coef[3 * pl + o];
+    const float uH4 = d.coef[4 * pl + o];
+    const float idx = 1.0f / d.dx;
+    int xq = -1, xk = 0, yq = -1, yk = 0;
+    if (P > 0) {
+        if (i < P) { xq = P - 1 - i; xk = HX0; }
+        else if (i >= NI - P) { xq = i - (NI - P); xk = HXM; }
+        if (j < P) { yq = P - 1 - j; yk = HY0; }
+        else if (j >= NJ - P) { yq = j - (NJ - P); yk = HYM; }
+    }
+    for (int s = 0; s < d.S; ++s) {
+        const size_t b = (size_t)s * pl + o;
+        float hy = d.Hy[b] + (x.U[b] - x.U[b + pitch]) + (x.Lx[b] - x.Lx[b + pitch]) * idx;
+        float hx = d.Hx[b] + (x.U[b + 1] - x.U[b]) + (x.Ly[b] - x.Ly[b + 1]) * idx;
+        d.Hy[b] = hy;
+        d.Hx[b] = hx;
+        float mx = 0.f, my = 0.f;
+        if (xq >= 0) {
+            float* pm = psi_x(d, xk, s, xq, j);
+            const float m = *pm;
+            mx = uH4 * (d.pml[4 * P + xq] - 1.0f) * hy - d.pml[7 * P + xq] * m;
+            *pm = uH4 * d.pml[5 * P + xq] * hy + d.pml[6 * P + xq] * m;
+        }
+        if (yq >= 0) {
+            float* pm = psi_y(d, yk, s, i, yq);
+            const float m = *pm;
+            my = -uH4 * (d.pml[4 * P + yq] - 1.0f) * hx - d.pml[7 * P + yq] * m;
+            *pm = -uH4 * d.pml[5 * P + yq] * hx + d.pml[6 * P + yq] * m;
+        }
+        x.Mx[b] = mx;
+        x.My[b] = my;
+        x.Vx[b] = i >= 1 ? uH1 * hx : 0.f;
+        x.Vy[b] = j >= 1 ? uH1 * hy : 0.f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlockJ* kBlockI)
+k_xadj_c(Dev d, float* scratch) {
+    const int j = blockIdx.x * kBlockJ + threadIdx.x;
+    const int i = blockIdx.y * kBlockI + threadIdx.y;
+    const int NI = d.NI, NJ = d.NJ, pitch = d.pitch;
+    if (i >= NI || j >= NJ) return;
+    const XScratch x = xscratch(d, scratch);
+    const size_t pl = plane(d);
+    const size_t o = (size_t)i * pitch + j;
+    const float c0 = (i >= 1 && j >= 1) ? d.coef[o] : 1.0f;
+    const float idx = 1.0f / d.dx;
+    for (int s = 0; s < d.S; ++s) {
+        const size_t b = (size_t)s * pl + o;
+        float v = c0 * d.Ez[b] + x.Vx[b] - x.Vy[b] - x.Mx[b] * idx - x.My[b] * idx;
+        if (j >= 1) v = v - x.Vx[b - 1] + x.My[b - 1] * idx;
+        if (i >= 1) v = v + x.Vy[b - pitch] + x.Mx[b - pitch] * idx;
+        d.Ez[b] = v;
+    }
+}
+
+// lambda_E at the receivers += dL/dtraces[n] (receivers on i = NI or j = NJ
+// record a field that is identically zero: nothing to inject).
+__global__ void k_xadj_record(Dev d, int koff, int dir) {
+    const int n = step_index(d, koff, dir);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.S * d.nrx) return;
+    const int s = t / d.nrx, r = t - s * d.nrx;
+    const int ri = d.rx[2 * t], rj = d.rx[2 * t + 1];
+    if (ri < 0 || ri >= d.NI || rj >= d.NJ) return;
+    atomicAdd(&d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj],
+              d.resid[((size_t)s * d.nt + n) * d.nrx + r]);
+}
+
 __global__ void k_advance_ctr(int* ctr, int delta) {
     if (threadIdx.x == 0) ctr[0] = ctr[0] + delta;
 }
@@ -360,7 +527,8 @@ __global__ void k_set_ctr(int* ctr, int v) {
 
 // Deterministic single-workgroup sum of squares, then scaled FP32 output.
 __global__ void __launch_bounds__(1024) k_normalise(const double* __restrict__ g, int64_t n,
-                                                    double scale, float* __restrict__ out) {
+                                                    double scale, int normalise,
+                                                    float* __restrict__ out) {
     __shared__ double red[1024];
     double acc = 0.0;
     for (int64_t k = threadIdx.x; k < n; k += 1024) acc += g[k] * g[k];
@@ -370,7 +538,7 @@ __global__ void __launch_bounds__(1024) k_normalise(const double* __restrict__ g
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    const double inv = scale / sqrt(red[0]);
+    const double inv = normalise ? scale / sqrt(red[0]) : scale;
     for (int64_t k = threadIdx.x; k < n; k += 1024) out[k] = (float)(g[k] * inv);
 }
 
@@ -418,11 +586,13 @@ __device__ __forceinline__ int xcd_remap(int b, int nwg) {
 }
 
 template <int RI, int NT, int K, bool ADJ>
-__global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nsteps) {
+__global__ void __launch_bounds__(NT)
+k_tb(TBArgs a, int koff, int dir, int nsteps) {
     constexpr int W = NT / 64;
-    constexpr int CPT = RI / W;
+    constexpr int CPT = RI / W;          // rows owned by one thread (a contiguous block)
     constexpr int TI = RI - 2 * K - 1;
     constexpr int TJ = kTBCols - 2 * K - 1;
+    constexpr int kMaxList = 256;
     static_assert(RI % W == 0 && TI > 0 && TJ > 0, "bad time-blocking geometry");
     const Dev& d = a.d;
     const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
@@ -435,12 +605,16 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
     const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
     const int gi0 = ti * TI - K, gj0 = tj * TJ - K;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r0 = w * CPT;              // first region row of this thread
     const int gj = gj0 + lane;
 
+    // Published copies of the fields: neighbours read them from LDS, the
+    // owner keeps its own cells in registers.
     __shared__ float sEz[RI * 64 + 1];
     __shared__ float sHx[RI * 64 + 1];
     __shared__ float sHy[RI * 64 + 1];
     __shared__ float sP[8 * kPmax];
+    __shared__ int sList[kMaxList];
 
     for (int e = threadIdx.x; e < 8 * P; e += NT) sP[e] = d.pml[e];
     if (threadIdx.x == 0) { sEz[RI * 64] = 0.f; sHx[RI * 64] = 0.f; sHy[RI * 64] = 0.f; }
@@ -449,8 +623,8 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
     const size_t shot_off = (size_t)s * pl;
     const bool colx = (gj >= 0 && gj <= NJ);
     const bool colact = (gj >= 0 && gj < NJ);
+    const bool centre_col = lane >= K && lane < K + TJ;
 
-    // y-slab CPML membership depends on the column only.
     int yqh = -1, yqe = -1, ykh = HY0, yke = EY0;
     if (P > 0 && colact) {
         if (gj < P) { yqh = P - 1 - gj; ykh = HY0; }
@@ -468,20 +642,33 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
         ye[2] = d.pml[2 * P + yqe]; ye[3] = d.pml[3 * P + yqe];
     }
 
+    float ez[CPT], hx[CPT], hy[CPT];
     float cE0[CPT], cE1[CPT], cE4[CPT], cH1[CPT], cH4[CPT];
     float pHx[CPT], pHy[CPT], pEx[CPT], pEy[CPT];
-    int xqh[CPT], xqe[CPT];
     unsigned act = 0u;   // bit m: cell m is updated (0 <= i < NI, 0 <= j < NJ)
+
+    // x-slab index of row gi for H (one cell lower on the minus side) and E.
+    auto xq_h = [&](int gi) -> int {
+        return gi < P ? P - 1 - gi : (gi >= NI - P ? gi - (NI - P) : -1);
+    };
+    auto xq_e = [&](int gi) -> int {
+        return (gi >= 1 && gi <= P) ? P - gi : (gi >= NI - P ? gi - (NI - P) : -1);
+    };
+    const float* pin = a.psi_in;
+    const size_t xs = xslab_floats(d), ys = yslab_floats(d);
 
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
-        const int r = w + W * m, gi = gi0 + r;
+        const int r = r0 + m, gi = gi0 + r;
         const int o = r * 64 + lane;
         const bool ex = colx && gi >= 0 && gi <= NI;
         const size_t g = shot_off + (size_t)gi * pitch + gj;
-        sEz[o] = ex ? a.Ez_in[g] : 0.f;
-        sHx[o] = ex ? a.Hx_in[g] : 0.f;
-        sHy[o] = ex ? a.Hy_in[g] : 0.f;
+        ez[m] = ex ? a.Ez_in[g] : 0.f;
+        hx[m] = ex ? a.Hx_in[g] : 0.f;
+        hy[m] = ex ? a.Hy_in[g] : 0.f;
+        sEz[o] = ez[m];
+        sHx[o] = hx[m];
+        sHy[o] = hy[m];
         const size_t c = (size_t)gi * pitch + gj;
         cE0[m] = ex ? d.coef[c] : 0.f;
         cE1[m] = ex ? d.coef[pl + c] : 0.f;
@@ -490,26 +677,25 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
         cH4[m] = ex ? d.coef[4 * pl + c] : 0.f;
         const bool ac = colact && gi >= 0 && gi < NI;
         if (ac) act |= 1u << m;
-        xqh[m] = -1;
-        xqe[m] = -1;
-        if (ac && P > 0) {
-            if (gi < P) xqh[m] = P - 1 - gi;
-            else if (gi >= NI - P) xqh[m] = gi - (NI - P);
-            if (gi >= 1 && gi <= P) xqe[m] = P - gi;
-            else if (gi >= NI - P) xqe[m] = gi - (NI - P);
-        }
-        const float* pin = a.psi_in;
-        pHx[m] = xqh[m] >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xslab_floats(d) +
-                                   ((size_t)s * P + xqh[m]) * NJ + gj] : 0.f;
-        pEx[m] = xqe[m] >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xslab_floats(d) +
-                                   ((size_t)s * P + xqe[m]) * NJ + gj] : 0.f;
-        pHy[m] = (ac && yqh >= 0) ? pin[4 * xslab_floats(d) + (size_t)(ykh - 4) * yslab_floats(d) +
-                                        ((size_t)s * NI + gi) * P + yqh] : 0.f;
-        pEy[m] = (ac && yqe >= 0) ? pin[4 * xslab_floats(d) + (size_t)(yke - 4) * yslab_floats(d) +
-                                        ((size_t)s * NI + gi) * P + yqe] : 0.f;
+        const int qh = (ac && P > 0) ? xq_h(gi) : -1;
+        const int qe = (ac && P > 0) ? xq_e(gi) : -1;
+        pHx[m] = qh >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] : 0.f;
+        pEx[m] = qe >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] : 0.f;
+        pHy[m] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(ykh - 4) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
+        pEy[m] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(yke - 4) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
     }
+    // Receiver list of this (shot, tile): local cell packed as li*64+lj, plus
+    // the receiver number (forward: centre receivers; adjoint: region ones).
     const int lst = a.rx_ptr ? a.rx_ptr[s * ntiles + tile] : 0;
-    const int lend = a.rx_ptr ? a.rx_ptr[s * ntiles + tile + 1] : 0;
+    const int lcount = (a.rx_ptr ? a.rx_ptr[s * ntiles + tile + 1] : 0) - lst;
+    for (int e = threadIdx.x; e < lcount && e < kMaxList / 2; e += NT) {
+        const int t = a.rx_idx[lst + e];
+        const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
+        const bool ok = li >= 0 && li < RI && lj >= 0 && lj < 64;
+        sList[2 * e] = ok ? li * 64 + lj : -1;
+        sList[2 * e + 1] = t - (t / d.nrx) * d.nrx;
+    }
+    const int nlist = lcount < kMaxList / 2 ? lcount : kMaxList / 2;
     const float dx = d.dx;
     double ae[CPT], as_[CPT];
     float fnext[CPT];
@@ -517,121 +703,124 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
     for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; fnext[m] = 0.f; }
     __syncthreads();
 
-    // ---- half-steps (shared by forward and adjoint) ----
+    // H half-step on the owned cells; publishes Hx (read by lane+1's E
+    // update) and the block's last Hy row (read by the next block's first row).
     auto h_half = [&]() {
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
             if (!(act & (1u << m))) continue;
-            const int r = w + W * m, gi = gi0 + r;
+            const int r = r0 + m, gi = gi0 + r;
             const int o = r * 64 + lane;
-            const float ez = sEz[o];
+            const float e = ez[m];
             const float ezj = sEz[o + 1];
-            const float ezi = (r + 1 < RI) ? sEz[o + 64] : 0.f;
+            const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ((r + 1 < RI) ? sEz[o + 64] : 0.f);
             if (gi >= 1 || yqh >= 0) {
-                float hx = sHx[o];
-                if (gi >= 1) hx = hx - cH1[m] * (ezj - ez);
+                float v = hx[m];
+                if (gi >= 1) v = v - cH1[m] * (ezj - e);
                 if (yqh >= 0) {
-                    const float dd = (ezj - ez) / dx;
-                    hx = hx - cH4[m] * (yh[0] * dd + yh[1] * pHy[m]);
+                    const float dd = (ezj - e) / dx;
+                    v = v - cH4[m] * (yh[0] * dd + yh[1] * pHy[m]);
                     pHy[m] = yh[2] * pHy[m] - yh[3] * dd;
                 }
-                sHx[o] = hx;
+                hx[m] = v;
             }
-            if (gj >= 1 || xqh[m] >= 0) {
-                float hy = sHy[o];
-                if (gj >= 1) hy = hy + cH1[m] * (ezi - ez);
-                if (xqh[m] >= 0) {
-                    const int q = xqh[m];
-                    const float dd = (ezi - ez) / dx;
-                    hy = hy + cH4[m] * ((sP[4 * P + q] - 1.0f) * dd + sP[5 * P + q] * pHx[m]);
+            if (gj >= 1 || (P > 0 && xq_h(gi) >= 0)) {
+                float v = hy[m];
+                if (gj >= 1) v = v + cH1[m] * (ezi - e);
+                const int q = P > 0 ? xq_h(gi) : -1;
+                if (q >= 0) {
+                    const float dd = (ezi - e) / dx;
+                    v = v + cH4[m] * ((sP[4 * P + q] - 1.0f) * dd + sP[5 * P + q] * pHx[m]);
                     pHx[m] = sP[6 * P + q] * pHx[m] - sP[7 * P + q] * dd;
                 }
-                sHy[o] = hy;
+                hy[m] = v;
             }
+            sHx[o] = hx[m];
+            if (m == CPT - 1) sHy[o] = hy[m];
         }
     };
 
-    // E update of cell m (main + CPML in slab order x0, y0, xmax, ymax).
+    // E update of owned cell m (main + CPML in slab order x0, y0, xmax, ymax).
     auto e_cell = [&](int m, int r, int gi, int o) -> float {
-        float ez = sEz[o];
-        const float hy = sHy[o];
-        const float hx = sHx[o];
-        const float hyim = (gi >= 1 && r >= 1) ? sHy[o - 64] : 0.f;
+        float v = ez[m];
+        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1 && r >= 1) ? sHy[o - 64] : 0.f);
         const float hxjm = (gj >= 1 && lane >= 1) ? sHx[o - 1] : 0.f;
-        if (gi >= 1 && gj >= 1) ez = (cE0[m] * ez + cE1[m] * (hy - hyim)) - cE1[m] * (hx - hxjm);
-        const int q = xqe[m];
+        if (gi >= 1 && gj >= 1) v = (cE0[m] * v + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+        const int q = P > 0 ? xq_e(gi) : -1;
         const bool xminus = gi <= P;
         float dxd = 0.f, dyd = 0.f;
-        if (q >= 0) dxd = (hy - hyim) / dx;
-        if (yqe >= 0) dyd = (hx - hxjm) / dx;
+        if (q >= 0) dxd = (hy[m] - hyim) / dx;
+        if (yqe >= 0) dyd = (hx[m] - hxjm) / dx;
         if (q >= 0 && xminus) {
-            ez = ez + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
+            v = v + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
             pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
         }
         if (yqe >= 0 && yke == EY0) {
-            ez = ez - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
+            v = v - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
             pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
         }
         if (q >= 0 && !xminus) {
-            ez = ez + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
+            v = v + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
             pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
         }
         if (yqe >= 0 && yke == EYM) {
-            ez = ez - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
+            v = v - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
             pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
         }
-        return ez;
+        return v;
     };
 
     for (int k = 0; k < nsteps; ++k) {
         const int n = ADJ ? n0 - k : n0 + k;
         if (!ADJ) {
-            // receivers of this tile, sampled before the step
-            for (int e = lst + threadIdx.x; e < lend; e += NT) {
-                const int t = a.rx_idx[e];
-                const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
-                const int r = t - (t / d.nrx) * d.nrx;
-                if (li < 0 || li >= RI || lj < 0 || lj >= 64) continue;
-                d.traces[((size_t)s * d.nt + n) * d.nrx + r] = sEz[li * 64 + lj];
+            // receivers of this tile's centre, sampled before the step
+            for (int e = threadIdx.x; e < nlist; e += NT) {
+                const int cell = sList[2 * e];
+                if (cell >= 0)
+                    d.traces[((size_t)s * d.nt + n) * d.nrx + sList[2 * e + 1]] = sEz[cell];
             }
             h_half();
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < CPT; ++m) {
                 if (!(act & (1u << m))) continue;
-                const int r = w + W * m, gi = gi0 + r;
+                const int r = r0 + m, gi = gi0 + r;
                 const int o = r * 64 + lane;
-                float ez = e_cell(m, r, gi, o);
+                float v = e_cell(m, r, gi, o);
                 for (int kk = 0; kk < d.nsrc; ++kk) {
                     const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
-                    if (sc[0] == gi && sc[1] == gj) ez = ez - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
+                    if (sc[0] == gi && sc[1] == gj) v = v - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
                 }
-                sEz[o] = ez;
-                if (d.snaps && r >= K && r < K + TI && lane >= K && lane < K + TJ)
-                    d.snaps[(((size_t)n * d.S + s) * NI + gi) * pitch + gj] = ez;
+                ez[m] = v;
+                sEz[o] = v;
+                if (d.snaps && r >= K && r < K + TI && centre_col)
+                    d.snaps[snap_off(d, n, s, gi, gj)] = v;
             }
             __syncthreads();
         } else {
-            // adjoint source at the receivers inside the region (back.cu:24-37)
-            for (int e = lst + threadIdx.x; e < lend; e += NT) {
-                const int t = a.rx_idx[e];
-                const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
-                const int r = t - (t / d.nrx) * d.nrx;
-                if (li < 0 || li >= RI || lj < 0 || lj >= 64) continue;
-                atomicAdd(&sEz[li * 64 + lj], -d.resid[((size_t)s * d.nt + n) * d.nrx + r]);
+            // adjoint source (back.cu:24-37) at receivers inside the region,
+            // applied by the owner of the cell in list order
+            for (int e = 0; e < nlist; ++e) {
+                const int cell = sList[2 * e];
+                const int li = cell >> 6, lj = cell & 63;
+                if (cell >= 0 && lj == lane && li >= r0 && li < r0 + CPT) {
+                    const float g = d.resid[((size_t)s * d.nt + n) * d.nrx + sList[2 * e + 1]];
+#pragma unroll
+                    for (int m = 0; m < CPT; ++m)
+                        if (li == r0 + m) ez[m] = ez[m] - g;
+                }
             }
-            __syncthreads();
 #pragma unroll
             for (int m = 0; m < CPT; ++m) {
                 if (!(act & (1u << m))) continue;
-                const int r = w + W * m, gi = gi0 + r;
+                const int r = r0 + m, gi = gi0 + r;
                 const int o = r * 64 + lane;
                 const float lam = e_cell(m, r, gi, o);
+                ez[m] = lam;
                 sEz[o] = lam;
-                if (r >= K && r < K + TI && lane >= K && lane < K + TJ) {
-                    const size_t so = (((size_t)n * d.S + s) * NI + gi) * pitch + gj;
-                    const float fn = (k == 0) ? d.snaps[so] : fnext[m];
-                    const float fp = d.snaps[so - (size_t)d.S * NI * pitch];
+                if (r >= K && r < K + TI && centre_col) {
+                    const float fn = (k == 0) ? d.snaps[snap_off(d, n, s, gi, gj)] : fnext[m];
+                    const float fp = d.snaps[snap_off(d, n - 1, s, gi, gj)];
                     ae[m] += (double)((fn - fp) * lam);
                     as_[m] += (double)(fn * lam);
                     fnext[m] = fp;
@@ -644,26 +833,24 @@ __global__ void __launch_bounds__(NT) k_tb(TBArgs a, int koff, int dir, int nste
     }
 
     // ---- write back the tile centre ----
+    float* pout = a.psi_out;
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
-        const int r = w + W * m, gi = gi0 + r;
-        if (!(r >= K && r < K + TI && lane >= K && lane < K + TJ)) continue;
+        const int r = r0 + m, gi = gi0 + r;
+        if (!(r >= K && r < K + TI && centre_col)) continue;
         if (!(colx && gi >= 0 && gi <= NI)) continue;
-        const int o = r * 64 + lane;
         const size_t g = shot_off + (size_t)gi * pitch + gj;
-        a.Ez_out[g] = sEz[o];
-        a.Hx_out[g] = sHx[o];
-        a.Hy_out[g] = sHy[o];
-        float* pout = a.psi_out;
-        if (xqh[m] >= 0)
-            pout[(size_t)(gi < P ? HX0 : HXM) * xslab_floats(d) + ((size_t)s * P + xqh[m]) * NJ + gj] = pHx[m];
-        if (xqe[m] >= 0)
-            pout[(size_t)(gi <= P ? EX0 : EXM) * xslab_floats(d) + ((size_t)s * P + xqe[m]) * NJ + gj] = pEx[m];
-        if ((act & (1u << m)) && yqh >= 0)
-            pout[4 * xslab_floats(d) + (size_t)(ykh - 4) * yslab_floats(d) + ((size_t)s * NI + gi) * P + yqh] = pHy[m];
-        if ((act & (1u << m)) && yqe >= 0)
-            pout[4 * xslab_floats(d) + (size_t)(yke - 4) * yslab_floats(d) + ((size_t)s * NI + gi) * P + yqe] = pEy[m];
-        if (ADJ && gi < NI && gj < NJ) {
+        a.Ez_out[g] = ez[m];
+        a.Hx_out[g] = hx[m];
+        a.Hy_out[g] = hy[m];
+        if (!(act & (1u << m))) continue;
+        const int qh = P > 0 ? xq_h(gi) : -1;
+        const int qe = P > 0 ? xq_e(gi) : -1;
+        if (qh >= 0) pout[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] = pHx[m];
+        if (qe >= 0) pout[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] = pEx[m];
+        if (yqh >= 0) pout[4 * xs + (size_t)(ykh - 4) * ys + ((size_t)s * NI + gi) * P + yqh] = pHy[m];
+        if (yqe >= 0) pout[4 * xs + (size_t)(yke - 4) * ys + ((size_t)s * NI + gi) * P + yqe] = pEy[m];
+        if (ADJ) {
             double* g0 = a.gshot + ((size_t)s * NI + gi) * NJ + gj;
             double* g1 = g0 + (size_t)d.S * NI * NJ;
             *g0 += ae[m];
@@ -714,6 +901,18 @@ int launch_adjoint_step(const Dev& d, const fgpr_problem& p, int koff, int dir, 
     return FGPR_OK;
 }
 
+int launch_exact_step(const Dev& d, const fgpr_problem& p, int koff, int dir, hipStream_t st) {
+    const dim3 grid = cell_grid(p), blk(kBlockJ, kBlockI, 1);
+    hipLaunchKernelGGL(k_xadj_a, grid, blk, 0, st, d, koff, dir, p.scratch);
+    hipLaunchKernelGGL(k_xadj_b, grid, blk, 0, st, d, p.scratch);
+    hipLaunchKernelGGL(k_xadj_c, grid, blk, 0, st, d, p.scratch);
+    const int ninj = p.S * p.nrx;
+    if (ninj > 0)
+        hipLaunchKernelGGL(k_xadj_record, dim3((ninj + 255) / 256), dim3(256), 0, st, d, koff, dir);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
 // ---- time-blocked launches ----
 using TBKernel = void (*)(TBArgs, int, int, int);
 
@@ -727,6 +926,8 @@ TBKernel tb_kernel(int ri, int nt, int k, bool adj) {
     FGPR_TB_CASE(64, 512, 1) FGPR_TB_CASE(64, 512, 2) FGPR_TB_CASE(64, 512, 3)
     FGPR_TB_CASE(64, 512, 4) FGPR_TB_CASE(64, 512, 6) FGPR_TB_CASE(64, 512, 8)
     FGPR_TB_CASE(64, 256, 2) FGPR_TB_CASE(64, 256, 4) FGPR_TB_CASE(64, 256, 8)
+    FGPR_TB_CASE(32, 512, 2) FGPR_TB_CASE(32, 512, 4) FGPR_TB_CASE(32, 512, 6)
+    FGPR_TB_CASE(64, 1024, 2) FGPR_TB_CASE(64, 1024, 4) FGPR_TB_CASE(64, 1024, 6)
     return nullptr;
 }
 #undef FGPR_TB_CASE
@@ -768,6 +969,20 @@ TBArgs make_tb(const fgpr_problem& p, const Dev& d, int parity, bool adj) {
     return a;
 }
 
+// After an odd number of ping-pong launches the state sits in the partner
+// buffers; copy it back so every call leaves it in Ez/Hx/Hy/psi.
+int tb_settle(const fgpr_problem& p, int parity, hipStream_t st) {
+    if (!parity) return FGPR_OK;
+    const size_t fb = (size_t)p.S * (p.NI + 1) * p.pitch * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(p.Ez, p.Ez2, fb, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.Hx, p.Hx2, fb, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p.Hy, p.Hy2, fb, hipMemcpyDeviceToDevice, st));
+    if (p.P > 0)
+        HIP_TRY(hipMemcpyAsync(p.psi, p.psi2, (size_t)fgpr_psi_floats(p.NI, p.NJ, p.P, p.S) * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+    return FGPR_OK;
+}
+
 int launch_tb(const fgpr_problem& p, const Dev& d, bool adj, int parity, int koff, int dir,
               int nsteps, hipStream_t st) {
     TBKernel kfn = tb_kernel(p.tb_ri, p.tb_nt, p.tb_k, adj);
@@ -798,7 +1013,8 @@ GraphKey make_key(const fgpr_problem& p, int kind, int chunk) {
     return k;
 }
 
-// kind 0 = forward (counter advances by +chunk), kind 1 = adjoint (-chunk).
+// kind 0 = forward (counter advances by +chunk), 1 = reference adjoint,
+// 2 = exact adjoint (both -chunk).
 int get_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
     GraphKey key = make_key(p, kind, chunk);
@@ -817,7 +1033,9 @@ int get_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
         (void)hipStreamDestroy(cs);
         return fail(FGPR_EHIP, "hipStreamBeginCapture failed");
     }
-    if (tb_enabled(p)) {
+    if (kind == 2) {
+        for (int k = 0; k < chunk && rc == FGPR_OK; ++k) rc = launch_exact_step(d, p, k, dir, cs);
+    } else if (tb_enabled(p)) {
         // chunk is a multiple of 2*tb_k: an even number of ping-pong launches,
         // so every replay starts from the primary buffers.
         for (int c = 0; c * p.tb_k < chunk && rc == FGPR_OK; ++c)
@@ -902,6 +1120,11 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
     if (n0 < 0 || n1 > p->nt || n0 > n1) return fail(FGPR_EINVAL, "bad step range");
     if (!p->traces && p->nrx > 0) return fail(FGPR_EINVAL, "null traces");
     if (p->nsrc > 0 && (!p->src || !p->srcscale)) return fail(FGPR_EINVAL, "null source arrays");
+    if (p->snaps && p->snap_mod > 0 && p->nsnap < p->snap_mod)
+        return fail(FGPR_EINVAL, "snapshot ring larger than the buffer (snap_mod > nsnap)");
+    if (p->snaps && p->snap_mod == 0 &&
+        (n0 - p->snap_base < 0 || (n1 > n0 && n1 - 1 - p->snap_base >= p->nsnap)))
+        return fail(FGPR_EINVAL, "forward steps outside the snapshot buffer (snap_base, nsnap)");
     const bool tb = tb_enabled(*p);
     if (tb && (rc = tb_check(*p, false))) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -926,7 +1149,7 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
             parity ^= 1;
             n += ns;
         }
-        return FGPR_OK;
+        return tb_settle(q, parity, st);
     }
     for (; n < n1; ++n) {
         rc = launch_forward_step(d, q, n, 1, st);
@@ -939,10 +1162,38 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
                  void* stream) {
     int rc = check_problem(p);
     if (rc) return rc;
-    if (grad_mode != FGPR_GRAD_REFERENCE) return fail(FGPR_EINVAL, "unknown gradient mode");
-    if (n_lo < 1 || n_hi >= p->nt || n_hi < n_lo - 1) return fail(FGPR_EINVAL, "bad step range");
+    if (grad_mode != FGPR_GRAD_REFERENCE && grad_mode != FGPR_GRAD_EXACT)
+        return fail(FGPR_EINVAL, "unknown gradient mode");
+    const bool exact = grad_mode == FGPR_GRAD_EXACT;
+    if (n_lo < (exact ? 0 : 1) || n_hi >= p->nt || n_hi < n_lo - 1) return fail(FGPR_EINVAL, "bad step range");
     if (!p->snaps || !p->grad_eps || !p->grad_sig) return fail(FGPR_EINVAL, "adjoint needs snaps and gradients");
     if (p->nrx > 0 && (!p->resid || !p->rx)) return fail(FGPR_EINVAL, "null residual/receivers");
+    if (p->snap_mod > 0) {
+        if (p->nsnap < p->snap_mod || n_hi - (n_lo >= 1 ? n_lo - 1 : 0) + 1 > p->snap_mod)
+            return fail(FGPR_EINVAL, "adjoint range longer than the snapshot ring (snap_mod)");
+    } else {
+        const int lo = (n_lo >= 1 ? n_lo - 1 : 0) - p->snap_base, hi = n_hi - p->snap_base;
+        if (n_hi >= n_lo && (lo < 0 || hi >= p->nsnap))
+            return fail(FGPR_EINVAL, "adjoint steps outside the snapshot buffer (snap_base, nsnap)");
+    }
+    if (exact) {
+        if (!p->scratch) return fail(FGPR_EINVAL, "exact adjoint needs the scratch planes");
+        hipStream_t st = (hipStream_t)stream;
+        fgpr_problem q = *p;
+        int n = n_hi;
+        if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n_hi - n_lo + 1 >= chunk) {
+            hipGraphExec_t ex;
+            rc = get_graph(q, 2, chunk, &ex);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(64), 0, st, q.ctr, n_hi);
+            for (; n - chunk + 1 >= n_lo; n -= chunk) HIP_TRY(hipGraphLaunch(ex, st));
+        }
+        q.ctr = nullptr;
+        const Dev d = to_dev(q);
+        for (; n >= n_lo; --n)
+            if ((rc = launch_exact_step(d, q, n, 1, st))) return rc;
+        return FGPR_OK;
+    }
     const bool tb = tb_enabled(*p);
     if (tb && (rc = tb_check(*p, true))) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -967,6 +1218,7 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
             parity ^= 1;
             n -= ns;
         }
+        if ((rc = tb_settle(q, parity, st))) return rc;
         const int64_t ncell = (int64_t)p->NI * p->NJ;
         hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
                            p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);
@@ -982,7 +1234,7 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
 
 int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* stream) {
     if (!g || !out || n < 1) return fail(FGPR_EINVAL, "bad normalise arguments");
-    hipLaunchKernelGGL(k_normalise, dim3(1), dim3(1024), 0, (hipStream_t)stream, g, n, scale, out);
+    hipLaunchKernelGGL(k_normalise, dim3(1), dim3(1024), 0, (hipStream_t)stream, g, n, scale, 1, out);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }
@@ -1022,10 +1274,26 @@ int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void
             case FGPR_KERNEL_ADJ_H:
                 hipLaunchKernelGGL(k_h_phase<false>, grid, blk, 0, st, d, n, 1);
                 break;
+            case FGPR_KERNEL_TB_FWD:
+            case FGPR_KERNEL_TB_ADJ: {
+                const bool adj = kernel == FGPR_KERNEL_TB_ADJ;
+                if ((rc = tb_check(q, adj))) return rc;
+                if (adj && (n < q.tb_k || !q.snaps)) return fail(FGPR_EINVAL, "adjoint block needs n >= tb_k and snaps");
+                if (!adj && n + q.tb_k > q.nt) return fail(FGPR_EINVAL, "forward block runs past nt");
+                if ((rc = launch_tb(q, d, adj, r & 1, n, 1, q.tb_k, st))) return rc;
+                break;
+            }
             default:
                 return fail(FGPR_EINVAL, "unknown kernel id");
         }
     }
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* stream) {
+    if (!g || !out || n < 1) return fail(FGPR_EINVAL, "bad scale arguments");
+    hipLaunchKernelGGL(k_normalise, dim3(1), dim3(1024), 0, (hipStream_t)stream, g, n, scale, 0, out);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

@@ -19,11 +19,13 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 2
+ABI_VERSION = 5
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
-KERNEL_FWD_H, KERNEL_FWD_E, KERNEL_ADJ_INJECT, KERNEL_ADJ_E, KERNEL_ADJ_H = range(5)
+GRAD_EXACT = 1
+(KERNEL_FWD_H, KERNEL_FWD_E, KERNEL_ADJ_INJECT, KERNEL_ADJ_E, KERNEL_ADJ_H,
+ KERNEL_TB_FWD, KERNEL_TB_ADJ) = range(7)
 
 
 class Problem(ctypes.Structure):
@@ -47,6 +49,8 @@ class Problem(ctypes.Structure):
         ("tb_rx_ptr", ctypes.c_void_p), ("tb_rx_idx", ctypes.c_void_p),
         ("tb_inj_ptr", ctypes.c_void_p), ("tb_inj_idx", ctypes.c_void_p),
         ("gshot", ctypes.c_void_p),
+        ("scratch", ctypes.c_void_p),
+        ("snap_base", ctypes.c_int32), ("nsnap", ctypes.c_int32), ("snap_mod", ctypes.c_int32),
     ]
 
 
@@ -70,6 +74,8 @@ SIGNATURES = {
                                       ctypes.c_void_p, ctypes.c_void_p]),
     "fgpr_profile_kernel": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_void_p]),
+    "fgpr_scale": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                  ctypes.c_void_p, ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
 }
 

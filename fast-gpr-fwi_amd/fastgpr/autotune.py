@@ -18,7 +18,9 @@ from . import solver as _solver
 
 CANDIDATES = ([(32, 256, k) for k in (1, 2, 3, 4, 6)]
               + [(64, 512, k) for k in (1, 2, 3, 4, 6, 8)]
-              + [(64, 256, k) for k in (2, 4, 8)])
+              + [(64, 256, k) for k in (2, 4, 8)]
+              + [(32, 512, k) for k in (2, 4, 6)]
+              + [(64, 1024, k) for k in (2, 4, 6)])
 
 _cache: dict = {}
 
@@ -71,9 +73,9 @@ def choose(G, er, se, mr, candidates=None, steps: int = 48) -> tuple:
     for tb in candidates or CANDIDATES:
         try:
             _native.tb_tiles(G.NI, G.NJ, *tb)
-        except RuntimeError:
+            results[tb] = time_variant(G, er, se, mr, tb, steps)
+        except (RuntimeError, ValueError):
             continue
-        results[tb] = time_variant(G, er, se, mr, tb, steps)
     best = min(results, key=results.get)
     _cache[key] = best
     _save_file()

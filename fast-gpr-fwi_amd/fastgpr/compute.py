@@ -14,6 +14,14 @@ Extra keyword arguments (all default to the reference behaviour):
 * ``time_block``  -- None (one H and one E kernel per step), "auto" (the
                      autotuner picks a time-blocked variant for this grid) or
                      an explicit (region rows, threads, steps per launch).
+* ``gradient``    -- "reference" (the reference's normalised approximate
+                     gradient) or "exact" (dL/d er, dL/d se of the discrete
+                     forward map through its exact adjoint, unnormalised).
+* ``checkpoint``  -- None (store every snapshot, as the reference), an
+                     interval C (keep the solver state every C steps and
+                     recompute snapshots segment by segment in the adjoint),
+                     or "auto" (checkpoint only when the history would not
+                     fit in half the free device memory).
 """
 from __future__ import annotations
 
@@ -22,6 +30,7 @@ import time
 import torch
 from torch import Tensor
 
+from . import checkpoint as _ckpt
 from . import grid as _grid
 from . import solver as _solver
 from .grid import FDTDGrid
@@ -53,7 +62,8 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             receiver_location=None, receiver_step=None, pmlthick=0, freq=None,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
-            exec_mode="graph", graph_chunk=32, time_block=None):
+            exec_mode="graph", graph_chunk=32, time_block=None, gradient="reference",
+            checkpoint=None):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
     Same arguments and meaning as compute.py:33-50: ``nx`` is the depth
@@ -67,7 +77,8 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
                        source_step, receiver_location, receiver_step, pmlthick,
                        source_apmlitudes, step, er, se, mr, nsrc, nrx, dt, pml_formulation,
                        freq, total_step, tv, Fixedreceiver, nseg, customsource, customreceiver,
-                       exec_mode=exec_mode, graph_chunk=graph_chunk, time_block=time_block)
+                       exec_mode=exec_mode, graph_chunk=graph_chunk, time_block=time_block,
+                       gradient=gradient, checkpoint=checkpoint)
     if G.time_block == "auto":
         from .autotune import choose
         G.time_block = choose(G, er, se, mr)
@@ -79,7 +90,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                    receiver_location, receiver_step, pmlthick, source_apmlitudes, step, er, se,
                    mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
                    nseg, customsource=None, customreceiver=None, exec_mode="graph",
-                   graph_chunk=32, time_block=None) -> FDTDGrid:
+                   graph_chunk=32, time_block=None, gradient="reference",
+                   checkpoint=None) -> FDTDGrid:
     """Build the solve description (compute.py:726-872).
 
     ``nseg`` (the reference's unfinished segmenting, compute.py:543-553) and
@@ -95,6 +107,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
         raise ValueError("2-D TMz supports z-polarised sources only")
     if exec_mode not in ("graph", "eager"):
         raise ValueError("exec_mode must be 'graph' or 'eager'")
+    if gradient not in ("reference", "exact"):
+        raise ValueError("gradient must be 'reference' or 'exact'")
     NI, NJ = int(nx), int(ny)
     if int(nz) != 1:
         raise ValueError("2-D TMz solver: nz must be 1")
@@ -120,7 +134,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                     dx=dx, dt=Gdt, iterations=nt, pmlthick=P, source=src, receiver=rx,
                     srcwaves=srcwaves, pml=pml, tv=tv, exec_mode=exec_mode,
                     graph_chunk=int(graph_chunk),
-                    time_block=time_block if time_block in (None, "auto") else tuple(time_block))
+                    time_block=time_block if time_block in (None, "auto") else tuple(time_block),
+                    options={"gradient": gradient, "checkpoint": checkpoint})
 
 
 class GPRFWI(torch.autograd.Function):
@@ -132,8 +147,14 @@ class GPRFWI(torch.autograd.Function):
         ctx.save_for_backward(er, se, mr)
         ctx.G = G
         need_grad = er.requires_grad or se.requires_grad
-        traces, batch = _solver.forward(G, er, se, mr, store=need_grad)
-        ctx.batch = batch if need_grad else None
+        C = _ckpt.interval(G, G.options.get("checkpoint")) if need_grad else None
+        ctx.fw = None
+        if C:
+            traces, ctx.fw = _ckpt.forward(G, er, se, mr, C)
+            ctx.batch = None
+        else:
+            traces, batch = _solver.forward(G, er, se, mr, store=need_grad)
+            ctx.batch = batch if need_grad else None
         if torch.isnan(traces).any():
             print("⚠️ rxs_gpu contains NaN!")
         return traces
@@ -144,9 +165,17 @@ class GPRFWI(torch.autograd.Function):
             print("⚠️ grad_output contains NaN!")
         er, se, mr = ctx.saved_tensors
         G = ctx.G
-        g_er, g_se, _, _ = _solver.adjoint(G, ctx.batch, er, se, mr, grad_output,
-                                           want_er=er.requires_grad, want_se=se.requires_grad)
+        exact = G.options.get("gradient") == "exact"
+        if ctx.fw is not None:
+            g_er, g_se, _, _ = _ckpt.adjoint(G, ctx.fw, er, se, mr, grad_output,
+                                             want_er=er.requires_grad, want_se=se.requires_grad,
+                                             exact=exact)
+        else:
+            sweep = _solver.adjoint_exact if exact else _solver.adjoint
+            g_er, g_se, _, _ = sweep(G, ctx.batch, er, se, mr, grad_output,
+                                     want_er=er.requires_grad, want_se=se.requires_grad)
         ctx.batch = None
+        ctx.fw = None
         if G.tv:
             from .tv import total_variation
             t0 = time.time()

@@ -49,6 +49,9 @@ def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device) -> 
     return coef
 
 
+TB_MAX_LIST = 128   # receivers one time-blocked workgroup can hold (kMaxList / 2)
+
+
 def _csr(keys: np.ndarray, vals: np.ndarray, nkeys: int):
     order = np.argsort(keys, kind="stable")
     ptr = np.zeros(nkeys + 1, np.int32)
@@ -87,6 +90,10 @@ def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
                         av.append(ent)
     fptr, fidx = _csr(np.asarray(fk, np.int64), np.asarray(fv, np.int64), S * ntiles)
     aptr, aidx = _csr(np.asarray(ak, np.int64), np.asarray(av, np.int64), S * ntiles)
+    most = max(int(np.diff(fptr).max(initial=0)), int(np.diff(aptr).max(initial=0)))
+    if most > TB_MAX_LIST:
+        raise ValueError(f"time-blocked variant {tuple(tb)}: {most} receivers in one tile region "
+                         f"(at most {TB_MAX_LIST}); use smaller regions or the per-step kernels")
     return dict(TI=TI, TJ=TJ, ti=ti, tj=tj, fptr=fptr, fidx=fidx, aptr=aptr, aidx=aidx)
 
 
@@ -162,6 +169,9 @@ class Batch:
         p.psi = self.psi.data_ptr()
         p.traces = self.traces.data_ptr()
         p.snaps = self.snaps.data_ptr() if self.snaps is not None else None
+        p.snap_base = 0
+        p.snap_mod = 0
+        p.nsnap = self.snaps.shape[0] if self.snaps is not None else 0
         p.resid = resid.data_ptr() if resid is not None else None
         p.grad_eps = grad_eps.data_ptr() if grad_eps is not None else None
         p.grad_sig = grad_sig.data_ptr() if grad_sig is not None else None
@@ -186,26 +196,74 @@ def source_scale(G) -> torch.Tensor:
     return ((w * d) / ((d * d) * d)).contiguous()
 
 
+def emitting_sources(G, b: Batch) -> torch.Tensor:
+    """Device injection amounts of the emitting source rows; drops rows whose
+    waveform is identically zero from ``b`` (they inject exactly nothing:
+    Ez - uE4*0 == Ez)."""
+    scale = source_scale(G)
+    emit = (scale != 0).any(dim=1)
+    if not bool(emit.all()):
+        b.src = b.src[:, emit.to(b.src.device)].contiguous()
+        b.nsrc = int(b.src.shape[1])
+        scale = scale[emit]
+    scale = scale.contiguous().to(G.device)
+    b.srcscale = scale
+    return scale
+
+
 def forward(G, er, se, mr, store: bool = True):
     """Forward sweep over all nt steps.  Returns (traces (S, nt, nrx), batch)."""
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
     coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
     b = Batch(G, store=store, tb=G.time_block)
-    scale = source_scale(G)
-    # Rows whose waveform is identically zero inject exactly nothing
-    # (Ez - uE4*0 == Ez); only emitting rows go to the device.
-    emit = (scale != 0).any(dim=1)
-    if not bool(emit.all()):
-        b.src = b.src[:, emit.to(b.src.device)].contiguous()
-        b.nsrc = int(b.src.shape[1])
-        scale = scale[emit]
-    scale = scale.contiguous().to(dev)
-    b.srcscale = scale
+    scale = emitting_sources(G, b)
     p = b.problem(coef, srcscale=scale)
     check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], G.graph_chunk,
                            _stream(dev)), "forward")
     return b.traces, b
+
+
+EPS0_KERNEL = 8.8541878128e-12   # the coefficient kernel's float constant (uc.cu:8)
+
+
+def adjoint_exact(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
+    """Exact gradient dL/d er, dL/d se of the FP32 forward map (transpose of
+    the forward stepping), unnormalised, float32 (NI, NJ, 1)."""
+    dev = G.device
+    se_f = torch.clamp(se.detach(), min=0, max=1e6)
+    coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
+    b.reset_state()
+    resid = grad_output.detach().to(device=dev, dtype=torch.float32).contiguous()
+    if resid.shape != (b.S, b.nt, b.nrx):
+        raise ValueError(f"grad_output shape {tuple(resid.shape)} != {(b.S, b.nt, b.nrx)}")
+    NI, NJ = b.NI, b.NJ
+    ge = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    gs = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    scratch = torch.zeros((7, b.S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
+    p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
+    p.tb_k = 0
+    p.scratch = scratch.data_ptr()
+    check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 0, _native.GRAD_EXACT,
+                           _MODES[G.exec_mode], G.graph_chunk, _stream(dev)), "exact adjoint")
+    return finish_exact(G, ge, gs, want_er, want_se)
+
+
+def finish_exact(G, ge, gs, want_er, want_se):
+    """dL/d er = -(e0/dt) * sum uE4*lambda*dE, dL/d se = the sigma sum, as
+    float32 (NI, NJ, 1); plus the raw FP64 sums."""
+    dev = G.device
+    NI, NJ = ge.shape
+    scale_e = -float(np.float32(EPS0_KERNEL)) / G.dt32
+    out = []
+    for want, g, sc in ((want_er, ge, scale_e), (want_se, gs, 1.0)):
+        if not want:
+            out.append(None)
+            continue
+        o = torch.empty((NI, NJ, 1), dtype=torch.float32, device=dev)
+        check(lib.fgpr_scale(_ptr(g), NI * NJ, sc, _ptr(o), _stream(dev)), "scale")
+        out.append(o)
+    return out[0], out[1], ge, gs
 
 
 def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
@@ -229,6 +287,14 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     if b.nt >= 2:
         check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 1, _native.GRAD_REFERENCE,
                                _MODES[G.exec_mode], G.graph_chunk, _stream(dev)), "adjoint")
+    return finish_reference(G, ge, gs, want_er, want_se)
+
+
+def finish_reference(G, ge, gs, want_er, want_se):
+    """g / ||g||_2 per requested parameter, float32 (NI, NJ, 1)
+    (compute.py:1174-1180); plus the raw FP64 sums."""
+    dev = G.device
+    NI, NJ = ge.shape
     out = []
     for want, g in ((want_er, ge), (want_se, gs)):
         if not want:

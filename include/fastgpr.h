@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 2
+#define FGPR_ABI_VERSION 5
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -58,6 +58,7 @@ extern "C" {
 
 /* Gradient modes for fgpr_adjoint. */
 #define FGPR_GRAD_REFERENCE 0  /* reference's approximate gradient (compute.py:1167-1172) */
+#define FGPR_GRAD_EXACT 1      /* exact discrete adjoint of the forward map (no reference counterpart) */
 
 /*
  * One batch of S shots sharing one model and one grid.
@@ -76,8 +77,9 @@ extern "C" {
  *             E-x0, E-xmax, H-x0, H-xmax  (each S*P*NJ, element [s][q][j]),
  *             E-y0, E-ymax, H-y0, H-ymax  (each S*NI*P, element [s][i][q]).
  *   traces    S*nt*nrx floats, element [s][n][r]        (forward output)
- *   snaps     nt*S*NI*pitch floats, element [n][s][i][j] (forward output,
- *             adjoint input); NULL = do not store.
+ *   snaps     nsnap*S*NI*pitch floats, element [slot(n)][s][i][j] with
+ *             slot(n) = n - snap_base, or n % snap_mod when snap_mod > 0
+ *             (forward output, adjoint input); NULL = do not store.
  *   resid     S*nt*nrx floats: dL/dtraces (adjoint input, back.cu:30).
  *   grad_eps, grad_sig   NI*NJ doubles, accumulated (+=) by fgpr_adjoint.
  *   ctr       one device int32 used as the step counter of captured graphs.
@@ -112,6 +114,11 @@ typedef struct fgpr_problem {
     const int32_t* tb_inj_ptr;  /* adjoint: receivers inside each (shot, tile) region */
     const int32_t* tb_inj_idx;
     double* gshot;              /* adjoint: 2*S*NI*NJ per-shot FP64 partial gradients (zeroed by caller) */
+    float* scratch;             /* exact adjoint: 7*S*(NI+1)*pitch floats, zeroed by the caller */
+    int32_t snap_base;          /* step of snapshot slot 0 (checkpointed segments; 0 otherwise) */
+    int32_t nsnap;              /* snapshot slots in `snaps` (nt when it holds every step) */
+    int32_t snap_mod;           /* > 0: step n lives in slot n % snap_mod (a ring of segment
+                                   slots; the same captured graph then serves every segment) */
 } fgpr_problem;
 
 /* ABI version (FGPR_ABI_VERSION). */
@@ -133,7 +140,7 @@ int64_t fgpr_psi_floats(int NI, int NJ, int P, int S);
  * Tile geometry of a time-blocked variant: centre tile size (*TI x *TJ) and
  * tile counts for an NI x NJ grid.  Returns FGPR_EINVAL for a variant that
  * is not compiled in.  Supported (tb_ri, tb_nt): (32, 256), (64, 512),
- * (64, 256); tb_k up to 8.
+ * (64, 256), (32, 512), (64, 1024); tb_k up to 8.
  */
 int fgpr_tb_tiles(int NI, int NJ, int tb_ri, int tb_nt, int tb_k, int* TI, int* TJ,
                   int* tiles_i, int* tiles_j);
@@ -158,10 +165,16 @@ int fgpr_cell_coeffs(const float* er, const float* se, const float* mr,
 int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, void* stream);
 
 /*
- * Adjoint steps n = n_hi down to n_lo (inclusive, n_lo >= 1), reference order
- * (compute.py:1048-1172): inject -resid at the receivers, E + E-CPML, H +
- * H-CPML, gradient accumulation into grad_eps / grad_sig (FP64, summed over
- * shots).  grad_sig is accumulated without its constant dt factor.
+ * Adjoint steps n = n_hi down to n_lo (inclusive).
+ * FGPR_GRAD_REFERENCE (n_lo >= 1): the reference's order (compute.py:1048-1172):
+ *   inject -resid at the receivers, E + E-CPML, H + H-CPML with the Ucgeta
+ *   coefficients, accumulation of sum (Ez[n]-Ez[n-1])*lambda into grad_eps and
+ *   sum Ez[n]*lambda into grad_sig (FP64, summed over shots; the constant dt of
+ *   the sigma term is left out -- it cancels in the normalisation).
+ * FGPR_GRAD_EXACT (n_lo >= 0, pass the FORWARD coefficients): the transpose of
+ *   the forward stepping; grad_eps accumulates sum uE4*lambda*(E^{n+1}-E^n)
+ *   (dL/d er = -(e0/dt) times it) and grad_sig accumulates dL/d se directly.
+ *   Needs `scratch`.  psi / fields hold the adjoint state and must start at 0.
  */
 int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int mode,
                  int chunk, void* stream);
@@ -178,6 +191,8 @@ int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* s
 #define FGPR_KERNEL_ADJ_INJECT 2 /* adjoint source at the receivers */
 #define FGPR_KERNEL_ADJ_E 3      /* E + E-CPML + gradient accumulation */
 #define FGPR_KERNEL_ADJ_H 4      /* H + H-CPML (adjoint coefficients) */
+#define FGPR_KERNEL_TB_FWD 5     /* time-blocked forward block of tb_k steps from step n */
+#define FGPR_KERNEL_TB_ADJ 6     /* time-blocked adjoint block of tb_k steps down from step n */
 
 /*
  * Launch one solver kernel `reps` times back to back at step n, on the
@@ -186,6 +201,9 @@ int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* s
  * counterpart in the reference (measurement only).
  */
 int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void* stream);
+
+/* out[k] = (float)(scale * g[k]) over n doubles. */
+int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* stream);
 
 /* Release every HIP graph the library captured (also done at unload). */
 int fgpr_release_graphs(void);

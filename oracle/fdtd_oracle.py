@@ -383,6 +383,126 @@ def adjoint_reference(cadj: Coeffs, pml: dict, P: int, dx: float,
     return ge, gs
 
 
+def _slab_rows_e(side, P, n):
+    return _slab_rows("E-" if side in ("x0", "y0") else "E+", P, n)
+
+
+def _slab_rows_h(side, P, n):
+    return _slab_rows("H-" if side in ("x0", "y0") else "H+", P, n)
+
+
+def adjoint_exact(c: Coeffs, pml: dict, P: int, dx: float, dt: float, rx: np.ndarray,
+                  resid: np.ndarray, snaps: np.ndarray):
+    """Exact discrete adjoint (transpose) of ``forward`` in float64.
+
+    Returns (dL/d er, dL/d se) for dL/dtraces = ``resid`` through the
+    per-cell coefficients of ``cell_coeffs``, using
+        dE^{n+1}/d er = -(e0/dt) uE4 (E^{n+1} - E^n),
+        dE^{n+1}/d se = -uE4 (E^{n+1} + E^n)/2   (main-update cells),
+    which hold because every term of the E update other than uE0 E^n is
+    uE4 (or uE1 = uE4/dx) times something independent of the cell's own
+    er / se.  The CPML sigma_max dependence on the mean of er over row 0 is
+    not included (finite-difference checks keep row 0 fixed).
+    """
+    f = np.float64
+    S, nt, nrx = resid.shape
+    NI, NJ = c.uE0.shape[0] - 1, c.uE0.shape[1] - 1
+    cc = Coeffs(*(np.asarray(a, f) for a in (c.uE0, c.uE1, c.uE4, c.uH0, c.uH1, c.uH4)))
+    fdx = f(dx)
+    e0dt = f(EPS0_KERNEL) / f(dt)
+    pm = {k: np.asarray(v, f) for k, v in pml.items() if k != "smax"} if P else {}
+    shp = (S, NI + 1, NJ + 1)
+    lE = np.zeros(shp)
+    lHx = np.zeros(shp)
+    lHy = np.zeros(shp)
+    mE = {sd: np.zeros((S, P, NJ) if sd[0] == "x" else (S, NI, P)) for sd in ("x0", "xmax", "y0", "ymax")}
+    mH = {sd: np.zeros((S, P, NJ) if sd[0] == "x" else (S, NI, P)) for sd in ("x0", "xmax", "y0", "ymax")}
+    ge = np.zeros((NI, NJ))
+    gs = np.zeros((NI, NJ))
+    ii, jj = np.meshgrid(np.arange(NI + 1), np.arange(NJ + 1), indexing="ij")
+    mainE = (ii >= 1) & (jj >= 1)
+    coefE = np.where(mainE, cc.uE0, 1.0)
+    hxmain = ii >= 1
+    hymain = jj >= 1
+    s_idx = np.repeat(np.arange(S), nrx)
+    ri = rx[:, :, 0].reshape(-1)
+    rj = rx[:, :, 1].reshape(-1)
+    res = np.asarray(resid, f)
+
+    def up(X):      # X[i+1]
+        Y = np.zeros_like(X); Y[:, :-1] = X[:, 1:]; return Y
+
+    def down(X):    # X[i-1]
+        Y = np.zeros_like(X); Y[:, 1:] = X[:, :-1]; return Y
+
+    def left(X):    # X[j+1]
+        Y = np.zeros_like(X); Y[:, :, :-1] = X[:, :, 1:]; return Y
+
+    def right(X):   # X[j-1]
+        Y = np.zeros_like(X); Y[:, :, 1:] = X[:, :, :-1]; return Y
+
+    for n in range(nt - 1, -1, -1):
+        e = lE
+        En1 = np.asarray(snaps[n], f)
+        En = np.asarray(snaps[n - 1], f) if n >= 1 else np.zeros_like(En1)
+        eC = e[:, :NI, :NJ]
+        u4 = cc.uE4[:NI, :NJ]
+        mC = mainE[:NI, :NJ]
+        ge += np.sum(-e0dt * u4 * eC * (En1 - En), axis=0)
+        gs += np.sum(np.where(mC, -u4 * eC * (En1 + En) / 2, -0.5 * u4 * eC * (En1 - En)), axis=0)
+        # transpose of the E half-step
+        U = np.where(mainE, cc.uE1 * e, 0.0)
+        Lx = np.zeros(shp)
+        Ly = np.zeros(shp)
+        if P:
+            RA1, RB, RE, RF = pm["ERA"] - 1, pm["ERB"], pm["ERE"], pm["ERF"]
+            for sd in ("x0", "xmax"):
+                rows = _slab_rows_e(sd, P, NI)
+                ec = e[:, rows, :NJ]
+                u = cc.uE4[rows, :NJ]
+                m = mE[sd]
+                Lx[:, rows, :NJ] += u * RA1[:, None] * ec - RF[:, None] * m
+                mE[sd] = u * RB[:, None] * ec + RE[:, None] * m
+            for sd in ("y0", "ymax"):
+                cols = _slab_rows_e(sd, P, NJ)
+                ec = e[:, :NI, cols]
+                u = cc.uE4[:NI, cols]
+                m = mE[sd]
+                Ly[:, :NI, cols] += -u * RA1[None, :] * ec - RF[None, :] * m
+                mE[sd] = -u * RB[None, :] * ec + RE[None, :] * m
+        lHy = lHy + U - up(U) + (Lx - up(Lx)) / fdx
+        lHx = lHx - U + left(U) + (Ly - left(Ly)) / fdx
+        lHx[:, NI, :] = 0
+        lHx[:, :, NJ] = 0
+        lHy[:, NI, :] = 0
+        lHy[:, :, NJ] = 0
+        # transpose of the H half-step
+        Vx = np.where(hxmain, cc.uH1 * lHx, 0.0)
+        Vy = np.where(hymain, cc.uH1 * lHy, 0.0)
+        Mx = np.zeros(shp)
+        My = np.zeros(shp)
+        if P:
+            RA1, RB, RE, RF = pm["HRA"] - 1, pm["HRB"], pm["HRE"], pm["HRF"]
+            for sd in ("x0", "xmax"):
+                rows = _slab_rows_h(sd, P, NI)
+                hc = lHy[:, rows, :NJ]
+                u = cc.uH4[rows, :NJ]
+                m = mH[sd]
+                Mx[:, rows, :NJ] += u * RA1[:, None] * hc - RF[:, None] * m
+                mH[sd] = u * RB[:, None] * hc + RE[:, None] * m
+            for sd in ("y0", "ymax"):
+                cols = _slab_rows_h(sd, P, NJ)
+                hc = lHx[:, :NI, cols]
+                u = cc.uH4[:NI, cols]
+                m = mH[sd]
+                My[:, :NI, cols] += -u * RA1[None, :] * hc - RF[None, :] * m
+                mH[sd] = -u * RB[None, :] * hc + RE[None, :] * m
+        lE = (coefE * e + Vx - right(Vx) - Vy + down(Vy)
+              + (down(Mx) - Mx) / fdx + (right(My) - My) / fdx)
+        np.add.at(lE, (s_idx, ri, rj), res[:, n, :].reshape(-1))
+    return ge, gs
+
+
 def normalise(g: np.ndarray) -> np.ndarray:
     """g / ||g||_2 (compute.py:1174-1180)."""
     return g / np.sqrt(np.sum(np.asarray(g, np.float64) ** 2))
@@ -415,6 +535,12 @@ def run_forward(pb: Problem, dtype=np.float64, store=True):
     c, pml = setup(pb, dtype)
     scale = source_scale(pb.wavelet, pb.dx, dtype)
     return forward(c, pml, pb.P, pb.dx, pb.src, scale, pb.rx, pb.nt, dtype, store)
+
+
+def run_gradient_exact(pb: Problem, resid: np.ndarray, snaps: np.ndarray):
+    """Exact (unnormalised) dL/d er, dL/d se of the FP64 forward map."""
+    c, pml = setup(pb, np.float64)
+    return adjoint_exact(c, pml, pb.P, pb.dx, pb.dt, pb.rx, resid, snaps)
 
 
 def run_gradient(pb: Problem, resid: np.ndarray, snaps: np.ndarray, dtype=np.float64):

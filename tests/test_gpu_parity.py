@@ -132,3 +132,60 @@ def test_autotuned_c1_gradient_matches_oracle(hip_device):
         e = rel_l2(g[k], o[k])
         print(f"C1 auto-tuned grad {k} rel-L2 vs fp64 oracle {e:.3e}")
         assert e <= GRAD_TOL_FP64
+
+
+def test_exact_gradient_matches_oracle_exact_adjoint(hip_device):
+    """gradient='exact': the HIP transpose of the FP32 forward stepping
+    against the oracle's FP64 exact adjoint (itself pinned by finite
+    differences and the dot-product test)."""
+    from fastgpr.workloads import small_case
+    from helpers import oracle_problem
+    from oracle import fdtd_oracle as O
+    wl = small_case(nt=200)
+    resid = _resid(wl)
+    _, g = run_product(wl, hip_device, resid=resid, gradient="exact")
+    pb = oracle_problem(wl)
+    _, sn = O.run_forward(pb)
+    ge, gs = O.run_gradient_exact(pb, resid.astype(np.float64), sn)
+    for k, o in (("er", ge), ("se", gs)):
+        e = rel_l2(g[k], o)
+        print(f"exact grad {k} rel-L2 vs fp64 oracle {e:.3e}")
+        assert e <= GRAD_TOL_FP64
+
+
+def test_exact_gradient_central_differences_16_cells(hip_device):
+    """Central differences of the GPU (FP32) forward map on 16 seeded cells
+    for each of er and se, loss = <r, traces> in FP64.  er uses a relative
+    step of 1e-3.  For se a 1e-3 step changes uE0 by ~1e-5 relative, which
+    float32 resolves to only ~0.6%, so se uses 5e-2 here; the 1e-3 check of
+    both parameters runs on the FP64 forward map in the oracle suite."""
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=200)
+    resid = _resid(wl).astype(np.float64)
+    _, g = run_product(wl, hip_device, resid=resid.astype(np.float32), gradient="exact")
+    rng = np.random.default_rng(3)
+    NI, NJ = wl.NI, wl.NJ
+    fd, ad = {"er": [], "se": []}, {"er": [], "se": []}
+    cells = []
+    while len(cells) < 16:
+        c = (int(rng.integers(1, NI)), int(rng.integers(0, NJ)))
+        if wl.se[c] < 100 and c not in cells:
+            cells.append(c)
+    for (i, j) in cells:
+        for k in ("er", "se"):
+            base = (wl.er if k == "er" else wl.se).astype(np.float64)
+            h = (1e-3 if k == "er" else 5e-2) * base[i, j]
+            vals = []
+            for sgn in (1, -1):
+                m = base.copy()
+                m[i, j] += sgn * h
+                model = (m, wl.se) if k == "er" else (wl.er, m)
+                tr, _ = run_product(wl, hip_device, model=model)
+                vals.append(np.sum(tr.astype(np.float64) * resid))
+            fd[k].append((vals[0] - vals[1]) / (2 * h))
+            ad[k].append(g[k][i, j])
+    for k in ("er", "se"):
+        a, f = np.array(ad[k]), np.array(fd[k])
+        e = rel_l2(a, f)
+        print(f"FD check {k}: rel-L2(adjoint, central differences) over 16 cells = {e:.3e}")
+        assert e <= 2e-2

@@ -287,3 +287,57 @@ def test_adjoint_reference_zero_on_dead_lines():
     assert np.abs(ge).max() > 0
     for g in (ge, gs):
         assert np.all(g[0, :] == 0) and np.all(g[:, 0] == 0)
+
+
+# --- exact adjoint: dot-product test and central finite differences (FP64) ----
+def _exact_case():
+    rng = np.random.default_rng(0)
+    NI, NJ, P = 30, 26, 5
+    er = rng.uniform(2, 8, (NI, NJ))
+    se = rng.uniform(1e-3, 5e-3, (NI, NJ))
+    mr = np.ones((NI, NJ))
+    dt = 0.9 * O.cfl_dt(0.05)
+    nt = 150
+    src = np.array([[[12, 9]], [[17, 15]]])
+    rx = np.array([[[20, 18], [6, 20], [6, 20]]] * 2)       # includes a colliding pair
+    w = O.ricker(nt, dt, 4e8)[None, :]
+
+    def F(er_, se_):
+        return O.run_forward(O.Problem(er_, se_, mr, 0.05, dt, nt, P, src, rx, w))
+
+    tr, sn = F(er, se)
+    r = rng.standard_normal(tr.shape)
+    ge, gs = O.run_gradient_exact(O.Problem(er, se, mr, 0.05, dt, nt, P, src, rx, w), r, sn)
+    return rng, er, se, F, r, ge, gs
+
+
+def test_exact_adjoint_dot_product_with_finite_differences():
+    """<J dm, r> by central differences of the forward map equals <dm, J^T r>
+    (row 0 is held fixed: it also enters the CPML sigma_max)."""
+    rng, er, se, F, r, ge, gs = _exact_case()
+    for name, g, base in (("er", ge, er), ("se", gs, se)):
+        d = rng.standard_normal(base.shape)
+        d[0, :] = 0
+        h = 1e-4 * np.abs(base).mean()
+        if name == "er":
+            fp, fm = F(er + h * d, se)[0], F(er - h * d, se)[0]
+        else:
+            fp, fm = F(er, se + h * d)[0], F(er, se - h * d)[0]
+        fd = np.sum((fp - fm) / (2 * h) * r)
+        ad = np.sum(d * g)
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (name, fd, ad)
+
+
+def test_exact_adjoint_single_cell_finite_differences():
+    rng, er, se, F, r, ge, gs = _exact_case()
+    cells = [(int(rng.integers(1, 30)), int(rng.integers(0, 26))) for _ in range(6)]
+    for (i, j) in cells:
+        for name, g, base in (("er", ge, er), ("se", gs, se)):
+            h = 1e-3 * base[i, j]
+            p = base.copy(); p[i, j] += h
+            m = base.copy(); m[i, j] -= h
+            if name == "er":
+                fd = np.sum((F(p, se)[0] - F(m, se)[0]) / (2 * h) * r)
+            else:
+                fd = np.sum((F(er, p)[0] - F(er, m)[0]) / (2 * h) * r)
+            assert abs(fd - g[i, j]) <= 1e-5 * max(abs(fd), 1e-12 * np.abs(g).max()), (name, i, j)
