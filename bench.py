@@ -43,12 +43,18 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=400,
                     help="time steps of the CPU oracle sample (forward + adjoint)")
     ap.add_argument("--profile-reps", type=int, default=200)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only launch a 256 MiB copy and the dominant kernels a few times "
+                         "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--traffic", default=str(Path(__file__).resolve().parent / "profiles" /
+                                             "traffic_c2_crosshole.json"),
+                    help="measured HBM bytes per launch (from the counter runs), if present")
     return ap.parse_args()
 
 
 def step_bytes(S: int, NI: int, NJ: int, P: int) -> float:
-    """Algorithmic HBM bytes of one time step of the batch (SURVEY 8(d) unit
-    figures): 40 B per cell-update (Ez/Hx/Hy read+write 24 B, three
+    """Algorithmic HBM bytes of one time step of the batch: 40 B per
+    cell-update (Ez/Hx/Hy read+write 24 B, three
     coefficients 12 B, snapshot write or read 4 B) plus 16 B per CPML cell
     and active direction, Npsi = 2P*NJ + 2P*NI (corners counted twice)."""
     return S * (40.0 * NI * NJ + 16.0 * (2 * P * NJ + 2 * P * NI))
@@ -96,6 +102,9 @@ def main():
 
     dev = torch.device("cuda:0")
     wl = c2_crosshole()
+    if args.pmc_probe:
+        pmc_probe(wl, dev, args)
+        return
     t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
     er_true, se_true, mr = t(wl.er), t(wl.se), t(wl.mr)
     er0, se0 = t(wl.er0), t(wl.se0)
@@ -149,6 +158,7 @@ def main():
         tb_used = tb
     roofline, per_kernel = profile_kernels(wl, er.detach(), se.detach(), mr, src, rx, tb_used,
                                            args.profile_reps)
+    roofline["traffic"] = measured_traffic(args.traffic, roofline["kernel"])
 
     # ---- CPU baseline: the numpy FP64 oracle on a bounded sample
     cpu = None
@@ -247,6 +257,49 @@ def profile_kernels(wl, er, se, mr, src, rx, tb, reps):
                 "kernel": best, "avg_launch_us": round(d["us"], 2),
                 "bytes_per_launch": int(d["bytes"])}
     return roofline, per
+
+
+def measured_traffic(path, kernel_name):
+    """HBM bytes per launch of ``kernel_name`` from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when no measurement of this kernel exists."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel_name.split(" ")[0])
+    return None if k is None else k.get("bytes_per_launch")
+
+
+def pmc_probe(wl, dev, args):
+    """Program for the FETCH_SIZE / WRITE_SIZE counter runs: a 256 MiB
+    device copy (the calibration: 256 MiB read + 256 MiB written) and the
+    autotuned variant's forward and adjoint kernels, 5 launches each."""
+    import numpy as np
+    import torch
+    import fastgpr
+    from fastgpr.autotune import _cache, _key
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t = lambda x: torch.tensor(np.asarray(x, np.float32))[..., None].to(dev)
+    er, se, mr = t(wl.er0), t(wl.se0), t(wl.mr)
+    src, rx = torch.tensor(wl.src), torch.tensor(wl.rx)
+    tb = (None if args.time_block == "none" else
+          "auto" if args.time_block == "auto" else tuple(int(v) for v in args.time_block.split(",")))
+    if tb == "auto":
+        Gk = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                                    None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                                    er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
+                                    False, False, None, src, rx)
+        from fastgpr.autotune import choose
+        tb = choose(Gk, er, se, mr)
+    profile_kernels(wl, er, se, mr, src, rx, tb, 5)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "time_block": list(tb) if tb else None}), flush=True)
 
 
 def plain_path(wl, er, se, mr, obs, src, rx, cu):

@@ -462,6 +462,13 @@ k_xadj_b(Dev d, float* scratch) {
         const size_t b = (size_t)s * pl + o;
         float hy = d.Hy[b] + (x.U[b] - x.U[b + pitch]) + (x.Lx[b] - x.Lx[b + pitch]) * idx;
         float hx = d.Hx[b] + (x.U[b + 1] - x.U[b]) + (x.Ly[b] - x.Ly[b + 1]) * idx;
+        // Hx on row 0 and Hy on column 0 belong to the closed, identically
+        // zero boundary subsystems of the forward map (only CPML terms update
+        // them); their adjoints never reach the interior, have a zero
+        // gradient weight, and the subsystem is not stable backwards, so
+        // they are held at zero.
+        if (i == 0) hx = 0.f;
+        if (j == 0) hy = 0.f;
         d.Hy[b] = hy;
         d.Hx[b] = hx;
         float mx = 0.f, my = 0.f;
@@ -500,7 +507,7 @@ k_xadj_c(Dev d, float* scratch) {
         float v = c0 * d.Ez[b] + x.Vx[b] - x.Vy[b] - x.Mx[b] * idx - x.My[b] * idx;
         if (j >= 1) v = v - x.Vx[b - 1] + x.My[b - 1] * idx;
         if (i >= 1) v = v + x.Vy[b - pitch] + x.Mx[b - pitch] * idx;
-        d.Ez[b] = v;
+        d.Ez[b] = (i >= 1 && j >= 1) ? v : 0.f;   // boundary subsystem, see k_xadj_b
     }
 }
 
@@ -586,8 +593,8 @@ __device__ __forceinline__ int xcd_remap(int b, int nwg) {
 }
 
 template <int RI, int NT, int K, bool ADJ>
-__global__ void __launch_bounds__(NT)
-k_tb(TBArgs a, int koff, int dir, int nsteps) {
+__device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int nsteps,
+                                        const int wg, const int nwg) {
     constexpr int W = NT / 64;
     constexpr int CPT = RI / W;          // rows owned by one thread (a contiguous block)
     constexpr int TI = RI - 2 * K - 1;
@@ -596,10 +603,9 @@ k_tb(TBArgs a, int koff, int dir, int nsteps) {
     static_assert(RI % W == 0 && TI > 0 && TJ > 0, "bad time-blocking geometry");
     const Dev& d = a.d;
     const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
-    const int n0 = step_index(d, koff, dir);
 
     const int ntiles = a.ti_count * a.tj_count;
-    const int L = xcd_remap(blockIdx.x, ntiles * d.S);
+    const int L = xcd_remap(wg, nwg);
     const int s = L % d.S;
     const int tile = L / d.S;
     const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
@@ -859,6 +865,42 @@ k_tb(TBArgs a, int koff, int dir, int nsteps) {
     }
 }
 
+template <int RI, int NT, int K, bool ADJ>
+__global__ void __launch_bounds__(NT)
+k_tb(TBArgs a, int koff, int dir, int nsteps) {
+    tb_body<RI, NT, K, ADJ>(a, step_index(a.d, koff, dir), nsteps, blockIdx.x,
+                            a.ti_count * a.tj_count * a.d.S);
+}
+
+// One launch over many problems (different grids, shots and time steps):
+// the table holds the workgroup prefix sums and one entry per problem.
+struct MultiEntry {
+    TBArgs a;
+    int n0;
+    int nsteps;
+    int pad[2];
+};
+
+__host__ __device__ inline size_t tb_table_header(int nitems) {
+    return (((size_t)(nitems + 1) * sizeof(int)) + 63) / 64 * 64;
+}
+
+template <int RI, int NT, int K, bool ADJ>
+__global__ void __launch_bounds__(NT)
+k_tb_multi(const unsigned char* __restrict__ table, int nitems) {
+    const int* prefix = (const int*)table;
+    const int b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const MultiEntry* ents = (const MultiEntry*)(table + tb_table_header(nitems));
+    const MultiEntry& me = ents[lo];
+    tb_body<RI, NT, K, ADJ>(me.a, me.n0, me.nsteps, b - prefix[lo], prefix[lo + 1] - prefix[lo]);
+}
+
 // grad[c] += sum_s gshot[p][s][c] in shot order (deterministic).
 __global__ void k_reduce_shots(const double* __restrict__ gshot, int S, int64_t ncell,
                                double* grad_eps, double* grad_sig) {
@@ -931,6 +973,21 @@ TBKernel tb_kernel(int ri, int nt, int k, bool adj) {
     return nullptr;
 }
 #undef FGPR_TB_CASE
+
+using TBMultiKernel = void (*)(const unsigned char*, int);
+
+#define FGPR_TBM_CASE(RI_, NT_, K_)                                                 \
+    if (ri == RI_ && nt == NT_ && k == K_)                                         \
+        return adj ? (TBMultiKernel)k_tb_multi<RI_, NT_, K_, true>                 \
+                   : (TBMultiKernel)k_tb_multi<RI_, NT_, K_, false>;
+
+TBMultiKernel tb_multi_kernel(int ri, int nt, int k, bool adj) {
+    FGPR_TBM_CASE(32, 512, 2) FGPR_TBM_CASE(32, 512, 4) FGPR_TBM_CASE(32, 512, 6)
+    FGPR_TBM_CASE(64, 1024, 2) FGPR_TBM_CASE(64, 1024, 4) FGPR_TBM_CASE(64, 1024, 6)
+    FGPR_TBM_CASE(64, 512, 4) FGPR_TBM_CASE(64, 512, 6)
+    return nullptr;
+}
+#undef FGPR_TBM_CASE
 
 bool tb_enabled(const fgpr_problem& p) { return p.tb_k > 0; }
 
@@ -1294,6 +1351,80 @@ int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void
 int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* stream) {
     if (!g || !out || n < 1) return fail(FGPR_EINVAL, "bad scale arguments");
     hipLaunchKernelGGL(k_normalise, dim3(1), dim3(1024), 0, (hipStream_t)stream, g, n, scale, 0, out);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int64_t fgpr_tb_table_bytes(int nitems) {
+    return nitems < 1 ? 0 : (int64_t)(tb_table_header(nitems) + (size_t)nitems * sizeof(MultiEntry));
+}
+
+int fgpr_tb_pack(const fgpr_tb_item* items, int nitems, int adj, void* host_table, int64_t bytes,
+                 int* total_wgs) {
+    if (!items || nitems < 1 || !host_table || !total_wgs) return fail(FGPR_EINVAL, "bad pack arguments");
+    if (bytes < fgpr_tb_table_bytes(nitems)) return fail(FGPR_EINVAL, "table buffer too small");
+    const fgpr_problem* p0 = items[0].prob;
+    if (!p0) return fail(FGPR_EINVAL, "null problem in item 0");
+    if (!tb_multi_kernel(p0->tb_ri, p0->tb_nt, p0->tb_k, adj != 0))
+        return fail(FGPR_EINVAL, "variant not compiled for multi-problem launches");
+    unsigned char* t = (unsigned char*)host_table;
+    int* prefix = (int*)t;
+    MultiEntry* ents = (MultiEntry*)(t + tb_table_header(nitems));
+    int64_t acc = 0;
+    for (int e = 0; e < nitems; ++e) {
+        const fgpr_problem* p = items[e].prob;
+        int rc = check_problem(p);
+        if (rc) return rc;
+        if (p->tb_ri != p0->tb_ri || p->tb_nt != p0->tb_nt || p->tb_k != p0->tb_k)
+            return fail(FGPR_EINVAL, "all items of one launch must share the time-blocking variant");
+        if ((rc = tb_check(*p, adj != 0))) return rc;
+        const int n0 = items[e].n0, ns = items[e].nsteps;
+        if (ns < 1 || ns > p->tb_k) return fail(FGPR_EINVAL, "nsteps must be in [1, tb_k]");
+        if (!adj && (n0 < 0 || n0 + ns > p->nt)) return fail(FGPR_EINVAL, "forward block outside [0, nt)");
+        if (adj && (n0 >= p->nt || n0 - ns + 1 < 1)) return fail(FGPR_EINVAL, "adjoint block outside [1, nt)");
+        if (adj && !p->snaps) return fail(FGPR_EINVAL, "adjoint block needs snapshots");
+        fgpr_problem q = *p;
+        q.ctr = nullptr;
+        prefix[e] = (int)acc;
+        acc += (int64_t)p->tb_ti * p->tb_tj * p->S;
+        if (acc > 0x7fffffff) return fail(FGPR_EINVAL, "too many workgroups in one launch");
+        MultiEntry me;
+        std::memset(&me, 0, sizeof(me));
+        me.a = make_tb(q, to_dev(q), items[e].parity & 1, adj != 0);
+        me.n0 = n0;
+        me.nsteps = ns;
+        std::memcpy(&ents[e], &me, sizeof(me));
+    }
+    prefix[nitems] = (int)acc;
+    *total_wgs = (int)acc;
+    return FGPR_OK;
+}
+
+int fgpr_tb_launch_table(const void* dev_table, int nitems, int total_wgs, int tb_ri, int tb_nt,
+                         int tb_k, int adj, void* stream) {
+    TBMultiKernel kfn = tb_multi_kernel(tb_ri, tb_nt, tb_k, adj != 0);
+    if (!kfn) return fail(FGPR_EINVAL, "variant not compiled for multi-problem launches");
+    if (!dev_table || nitems < 1 || total_wgs < 1) return fail(FGPR_EINVAL, "bad launch arguments");
+    hipLaunchKernelGGL(kfn, dim3(total_wgs), dim3(tb_nt), 0, (hipStream_t)stream,
+                       (const unsigned char*)dev_table, nitems);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int fgpr_tb_settle(const fgpr_problem* p, int parity, void* stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (!p->Ez2 || !p->Hx2 || !p->Hy2 || (p->P > 0 && !p->psi2)) return fail(FGPR_EINVAL, "no partner buffers");
+    return tb_settle(*p, parity & 1, (hipStream_t)stream);
+}
+
+int fgpr_reduce_shots(const fgpr_problem* p, void* stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (!p->gshot || !p->grad_eps || !p->grad_sig) return fail(FGPR_EINVAL, "reduce needs gshot and gradients");
+    const int64_t ncell = (int64_t)p->NI * p->NJ;
+    hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
@@ -54,6 +54,13 @@ class Problem(ctypes.Structure):
     ]
 
 
+class TBItem(ctypes.Structure):
+    """Mirror of ``fgpr_tb_item``."""
+
+    _fields_ = [("prob", ctypes.POINTER(Problem)), ("n0", ctypes.c_int32),
+                ("nsteps", ctypes.c_int32), ("parity", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); every symbol include/fastgpr.h declares.
 SIGNATURES = {
     "fgpr_abi_version": (ctypes.c_int, []),
@@ -76,6 +83,14 @@ SIGNATURES = {
                                            ctypes.c_int, ctypes.c_void_p]),
     "fgpr_scale": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
                                   ctypes.c_void_p, ctypes.c_void_p]),
+    "fgpr_tb_table_bytes": (ctypes.c_int64, [ctypes.c_int]),
+    "fgpr_tb_pack": (ctypes.c_int, [ctypes.POINTER(TBItem), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int)]),
+    "fgpr_tb_launch_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_void_p]),
+    "fgpr_tb_settle": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p]),
+    "fgpr_reduce_shots": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
 }
 

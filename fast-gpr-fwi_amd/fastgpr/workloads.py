@@ -9,6 +9,8 @@ stepping and acquisition as integer cell indices.  No data files are read.
 * ``c2_crosshole``  400 x 200 grid, dx 0.025 m, 3000 steps, Ricker 200 MHz,
                     13 sources down column 30, 80 receivers down column 170,
                     8 seeded layers + 3 inclusions.
+* ``c4_memory``     2000 x 1000 grid, dx 0.01 m, 8000 steps, 4 shots,
+                    200 receivers, Ricker 400 MHz (checkpointed gradients).
 * ``layered_blob``  generic layered + blob model of any size (uneven batches,
                     serving requests).
 """
@@ -156,3 +158,78 @@ def small_case(NI=60, NJ=44, P=8, S=3, nt=300, nrx=6, seed=5, dx=0.05, freq=2e8)
     rx = _cells(rx_pts)
     return Workload(f"small_{NI}x{NJ}", er.astype(np.float32), se.astype(np.float32), mr,
                     er.astype(np.float32), se.astype(np.float32), dx, dt, nt, P, freq, src, rx)
+
+
+def c4_memory(nt: int = 8000) -> Workload:
+    """2000 x 1000 grid at dx 0.01 m, 8000 steps, 4 shots, 200 receivers,
+    Ricker 400 MHz (>= 20 cells per shortest wavelength at eps <= 12).
+    Storing every snapshot would take ~64 GB per shot: the gradient uses
+    checkpointing."""
+    NI, NJ, P = 2000, 1000, 20
+    er, se = layered_blob(NI, NJ, seed=3, nlayers=10, nblobs=6)
+    dx = 0.01
+    dt = 0.9 * float(cfl_dt(dx))
+    src = _cells([[(i, 100)] for i in (400, 800, 1200, 1600)])
+    rx = _cells([[(100 + 9 * r, 900) for r in range(200)] for _ in range(4)])
+    mr = np.ones((NI, NJ))
+    return Workload("c4_memory_2000x1000_4shots", er.astype(np.float32), se.astype(np.float32),
+                    mr.astype(np.float32), _smooth(er, 20).astype(np.float32),
+                    _smooth(se, 20).astype(np.float32), dx, dt, nt, P, 4e8, src, rx)
+
+
+def crop(wl: Workload, i0: int, j0: int, NI: int, NJ: int, nt: int, shots=None,
+         name: str | None = None, receivers=None) -> Workload:
+    """A sub-grid of a workload (same dx, dt, CPML thickness and frequency),
+    keeping the sources that fall inside, re-indexed.  ``receivers`` (crop
+    cell indices, shared by every shot) replaces the receivers; otherwise the
+    ones inside the crop are kept."""
+    sl = (slice(i0, i0 + NI), slice(j0, j0 + NJ))
+    pick = range(wl.S) if shots is None else shots
+    src, rx = [], []
+    for s in pick:
+        ss = [(i - i0, j - j0) for i, j, _ in wl.src[s] if i0 <= i < i0 + NI and j0 <= j < j0 + NJ]
+        rr = ([tuple(p) for p in receivers] if receivers is not None else
+              [(i - i0, j - j0) for i, j, _ in wl.rx[s] if i0 <= i < i0 + NI and j0 <= j < j0 + NJ])
+        if ss and rr:
+            src.append(ss)
+            rx.append(rr)
+    if not rx:
+        raise ValueError("no shot has both a source and a receiver inside the crop")
+    n = min(len(r) for r in rx)
+    rx = [r[:n] for r in rx]
+    return Workload(name or f"{wl.name}_crop{NI}x{NJ}", wl.er[sl].copy(), wl.se[sl].copy(),
+                    wl.mr[sl].copy(), wl.er0[sl].copy(), wl.se0[sl].copy(), wl.dx, wl.dt, nt,
+                    wl.P, wl.freq, _cells(src), _cells(rx))
+
+
+def c3_uneven(seed: int = 2) -> list:
+    """64 shots over 4 seeded models of different sizes and step counts:
+    300x150/2000, 500x250/3000, 800x400/4500, 1200x600/6000 steps, dx 0.025 m,
+    Ricker 200 MHz, 20-cell CPML; 8-24 sources and 50-150 receivers per model,
+    layered + blob eps in [3, 15], sigma in [0.5, 20] mS/m."""
+    rng = np.random.default_rng(seed)
+    dims = [(300, 150, 2000), (500, 250, 3000), (800, 400, 4500), (1200, 600, 6000)]
+    while True:
+        shots = rng.integers(8, 25, size=4)
+        if shots.sum() == 64:
+            break
+    out = []
+    for m, ((NI, NJ, nt), S) in enumerate(zip(dims, shots)):
+        er, se = layered_blob(NI, NJ, seed=seed * 10 + m, eps=(3.0, 12.0), sig=(5e-4, 2e-2),
+                              nblobs=4)
+        er = np.clip(er, 3.0, 15.0)
+        se = np.clip(se, 5e-4, 2e-2)
+        nrx = int(rng.integers(50, 151))
+        dx = 0.025
+        dt = 0.9 * float(cfl_dt(dx))
+        sj = int(rng.integers(25, NJ // 4))
+        rj = int(rng.integers(3 * NJ // 4, NJ - 25))
+        si = np.round(np.linspace(30, NI - 31, int(S))).astype(int)
+        ri = np.round(np.linspace(30, NI - 31, nrx)).astype(int)
+        src = _cells([[(int(i), sj)] for i in si])
+        rx = _cells([[(int(i), rj) for i in ri] for _ in si])
+        out.append(Workload(f"c3_model{m}_{NI}x{NJ}_{int(S)}shots", er.astype(np.float32),
+                            se.astype(np.float32), np.ones((NI, NJ), np.float32),
+                            _smooth(er, 10).astype(np.float32), _smooth(se, 10).astype(np.float32),
+                            dx, dt, nt, 20, 2e8, src, rx))
+    return out

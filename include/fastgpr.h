@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 5
+#define FGPR_ABI_VERSION 6
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -204,6 +204,35 @@ int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void
 
 /* out[k] = (float)(scale * g[k]) over n doubles. */
 int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* stream);
+
+/*
+ * Multi-problem time-blocked launches (uneven batches, serving).  One launch
+ * advances many problems -- different grids, shot counts and time steps --
+ * by up to tb_k steps each; all items share (tb_ri, tb_nt, tb_k).  No
+ * counterpart in the reference (it runs one model per call).
+ *   fgpr_tb_pack        builds the launch table in host memory (validates
+ *                       every item; *total_wgs = workgroups of the launch);
+ *   fgpr_tb_launch_table launches it from a device copy of that table;
+ *   fgpr_tb_settle       copies a problem's state back from the partner
+ *                       buffers after an odd number of its blocks;
+ *   fgpr_reduce_shots    grad_eps/grad_sig += sum over shots of gshot.
+ * Supported variants: (32,512,{2,4,6}), (64,1024,{2,4,6}), (64,512,{4,6}).
+ */
+typedef struct fgpr_tb_item {
+    const fgpr_problem* prob;   /* host pointer to the problem descriptor */
+    int32_t n0;                 /* first step (forward: ascending, adjoint: descending) */
+    int32_t nsteps;             /* steps of this block, 1..tb_k */
+    int32_t parity;             /* 0: state in Ez/Hx/Hy/psi, 1: in the partner buffers */
+    int32_t pad;
+} fgpr_tb_item;
+
+int64_t fgpr_tb_table_bytes(int nitems);
+int fgpr_tb_pack(const fgpr_tb_item* items, int nitems, int adj, void* host_table, int64_t bytes,
+                 int* total_wgs);
+int fgpr_tb_launch_table(const void* dev_table, int nitems, int total_wgs, int tb_ri, int tb_nt,
+                         int tb_k, int adj, void* stream);
+int fgpr_tb_settle(const fgpr_problem* p, int parity, void* stream);
+int fgpr_reduce_shots(const fgpr_problem* p, void* stream);
 
 /* Release every HIP graph the library captured (also done at unload). */
 int fgpr_release_graphs(void);

@@ -476,6 +476,12 @@ def adjoint_exact(c: Coeffs, pml: dict, P: int, dx: float, dt: float, rx: np.nda
         lHx[:, :, NJ] = 0
         lHy[:, NI, :] = 0
         lHy[:, :, NJ] = 0
+        # Hx on row 0 / Hy on column 0 (and Ez on both lines) form the closed,
+        # identically zero boundary subsystems of the forward map: their
+        # adjoints never reach the interior and carry zero gradient weight,
+        # and that subsystem is unstable when run backwards, so it is held at 0.
+        lHx[:, 0, :] = 0
+        lHy[:, :, 0] = 0
         # transpose of the H half-step
         Vx = np.where(hxmain, cc.uH1 * lHx, 0.0)
         Vy = np.where(hymain, cc.uH1 * lHy, 0.0)
@@ -499,6 +505,8 @@ def adjoint_exact(c: Coeffs, pml: dict, P: int, dx: float, dt: float, rx: np.nda
                 mH[sd] = -u * RB[None, :] * hc + RE[None, :] * m
         lE = (coefE * e + Vx - right(Vx) - Vy + down(Vy)
               + (down(Mx) - Mx) / fdx + (right(My) - My) / fdx)
+        lE[:, 0, :] = 0
+        lE[:, :, 0] = 0
         np.add.at(lE, (s_idx, ri, rj), res[:, n, :].reshape(-1))
     return ge, gs
 
@@ -508,7 +516,7 @@ def normalise(g: np.ndarray) -> np.ndarray:
     return g / np.sqrt(np.sum(np.asarray(g, np.float64) ** 2))
 
 
-# --- convenience driver ------------------------------------------------------
+# --- problem set-up and runs ------------------------------------------------------
 @dataclass
 class Problem:
     er: np.ndarray          # (NI, NJ) relative permittivity

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through fastgpr.compute -> libfastgpr C ABI)
 against the oracle on identical inputs.
 
-Tolerances (BASELINE north star):
+Tolerances:
 * traces: <= 1e-5 relative L2 against the reference FP32 result (the oracle's
   FP32 mode, which follows the reference's FP32 operation order);
 * gradients: <= 1e-4 relative L2 against the FP64 oracle.

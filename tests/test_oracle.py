@@ -14,7 +14,7 @@ def test_cfl_dt_known_answer():
 
 
 def test_n_iterations_matches_notebook_run():
-    # example.ipynb: time_windows 1e-7, dt 1e-10 -> 1001 steps (BASELINE setup).
+    # example.ipynb: time_windows 1e-7, dt 1e-10 -> 1001 steps.
     assert O.n_iterations(1e-7, 1e-10) == 1001
 
 
