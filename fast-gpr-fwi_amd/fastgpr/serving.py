@@ -170,6 +170,8 @@ class Server:
 
     def step(self) -> bool:
         """Admit, advance one tick, reap.  Returns False when idle."""
+        if not self.stats.started:
+            self.stats.started = time.perf_counter()
         self._admit()
         busy = bool(self.sched.active())
         if busy:
@@ -185,8 +187,6 @@ class Server:
         return busy or bool(self.active) or bool(self.waiting) or not self.pending.empty()
 
     def run(self, stop_when_idle=True, until: float | None = None):
-        if not self.stats.started:
-            self.stats.started = time.perf_counter()
         while True:
             busy = self.step()
             now = time.perf_counter()
