@@ -31,6 +31,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "fastgpr.h"
@@ -555,17 +556,25 @@ __global__ void __launch_bounds__(1024) k_normalise(const double* __restrict__ g
 // A workgroup owns a TI x TJ tile of one shot and advances it up to K steps
 // per launch.  It loads the tile plus a halo -- a region of RI rows x 64
 // columns -- of Ez, Hx, Hy and the CPML memory variables from the "in"
-// buffers into LDS / registers, runs the steps in LDS, and writes the tile
-// centre to the "out" buffers (ping-pong, so neighbouring workgroups never
-// read cells another one is writing).  Each step the region's interior edges
-// go stale by one more cell; the halo of K cells on the low side and K + 1 on
-// the high side (H reads Ez at +1) absorbs that.  Domain edges are exact: the
-// region never extends past them and cells outside the domain read as zero.
-// Per-model coefficients live in registers for the whole launch; x-slab CPML
-// coefficients come from an LDS table, y-slab ones from registers (a thread
-// owns one column).
+// buffers, runs the steps on chip, and writes the tile centre to the "out"
+// buffers (ping-pong, so neighbouring workgroups never read cells another one
+// is writing).  Each step the region's interior edges go stale by one more
+// cell; the halo of K cells on the low side and K + 1 on the high side (H
+// reads Ez at +1) absorbs that.  Domain edges are exact: cells outside the
+// domain read as zero and are never updated.
+//
+// Lane = column, wave = a block of RI / W consecutive rows.  A thread keeps
+// its cells' fields and the three per-cell coefficients (cE0, cE1, cH1) in
+// registers for the whole launch; the column neighbours come from the
+// adjacent lane through DPP wave shifts, the row neighbours from the same
+// thread except at the wave's first / last row, which are exchanged through
+// two small LDS rows per wave.  Only the CPML memory variables (one LDS plane
+// per slab kind over the region), the CPML coefficient table and the
+// receiver lists live in LDS; cE4 / cH4 are read from the (L2-resident)
+// coefficient planes where a CPML cell or a source needs them.
 constexpr int kTBCols = 64;
 constexpr int kPmax = 128;
+constexpr int kWaveList = 128;   // receiver entries one wave can hold
 
 struct TBArgs {
     Dev d;
@@ -592,6 +601,15 @@ __device__ __forceinline__ int xcd_remap(int b, int nwg) {
     return x * q + (x < r ? x : r) + t;
 }
 
+__device__ __forceinline__ float lane_next(float v) {   // value of lane + 1 (0 on lane 63)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {   // value of lane - 1 (0 on lane 0)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x138, 0xf, 0xf, false));
+}
+
 template <int RI, int NT, int K, bool ADJ>
 __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int nsteps,
                                         const int wg, const int nwg) {
@@ -599,8 +617,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     constexpr int CPT = RI / W;          // rows owned by one thread (a contiguous block)
     constexpr int TI = RI - 2 * K - 1;
     constexpr int TJ = kTBCols - 2 * K - 1;
-    constexpr int kMaxList = 256;
-    static_assert(RI % W == 0 && TI > 0 && TJ > 0, "bad time-blocking geometry");
+    static_assert(RI % W == 0 && TI > 0 && TJ > 0 && CPT <= 16, "bad time-blocking geometry");
     const Dev& d = a.d;
     const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
 
@@ -610,20 +627,17 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const int tile = L / d.S;
     const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
     const int gi0 = ti * TI - K, gj0 = tj * TJ - K;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = w * CPT;              // first region row of this thread
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int r0 = w * CPT;              // first region row of this wave
     const int gj = gj0 + lane;
 
-    // Published copies of the fields: neighbours read them from LDS, the
-    // owner keeps its own cells in registers.
-    __shared__ float sEz[RI * 64 + 1];
-    __shared__ float sHx[RI * 64 + 1];
-    __shared__ float sHy[RI * 64 + 1];
-    __shared__ float sP[8 * kPmax];
-    __shared__ int sList[kMaxList];
-
-    for (int e = threadIdx.x; e < 8 * P; e += NT) sP[e] = d.pml[e];
-    if (threadIdx.x == 0) { sEz[RI * 64] = 0.f; sHx[RI * 64] = 0.f; sHy[RI * 64] = 0.f; }
+    // Rows meet between waves only: each wave publishes its first Ez row and
+    // last Hy row; columns meet inside a wave through lane shifts.
+    __shared__ float sEzTop[W * 64];
+    __shared__ float sHyBot[W * 64];
+    __shared__ float sPsi[4][RI * 64];   // CPML memory of the region: Hx-slab, Ex-slab, Hy-slab, Ey-slab
+    __shared__ int sRx[W][kWaveList];    // receivers of each wave: (m << 26) | (lane << 20) | r
 
     const size_t pl = plane(d);
     const size_t shot_off = (size_t)s * pl;
@@ -631,13 +645,15 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const bool colact = (gj >= 0 && gj < NJ);
     const bool centre_col = lane >= K && lane < K + TJ;
 
-    int yqh = -1, yqe = -1, ykh = HY0, yke = EY0;
+    int yqh = -1, yqe = -1;
+    bool yh_lo = true, ye_lo = true;
     if (P > 0 && colact) {
-        if (gj < P) { yqh = P - 1 - gj; ykh = HY0; }
-        else if (gj >= NJ - P) { yqh = gj - (NJ - P); ykh = HYM; }
-        if (gj >= 1 && gj <= P) { yqe = P - gj; yke = EY0; }
-        else if (gj >= NJ - P) { yqe = gj - (NJ - P); yke = EYM; }
+        if (gj < P) yqh = P - 1 - gj;
+        else if (gj >= NJ - P) { yqh = gj - (NJ - P); yh_lo = false; }
+        if (gj >= 1 && gj <= P) yqe = P - gj;
+        else if (gj >= NJ - P) { yqe = gj - (NJ - P); ye_lo = false; }
     }
+    // y-slab coefficients of this column (RA - 1, RB, RE, RF for H and E)
     float yh[4] = {0.f, 0.f, 0.f, 0.f}, ye[4] = {0.f, 0.f, 0.f, 0.f};
     if (yqh >= 0) {
         yh[0] = d.pml[4 * P + yqh] - 1.0f; yh[1] = d.pml[5 * P + yqh];
@@ -650,7 +666,6 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 
     float ez[CPT], hx[CPT], hy[CPT];
     float cE0[CPT], cE1[CPT], cE4[CPT], cH1[CPT], cH4[CPT];
-    float pHx[CPT], pHy[CPT], pEx[CPT], pEy[CPT];
     unsigned act = 0u;   // bit m: cell m is updated (0 <= i < NI, 0 <= j < NJ)
 
     // x-slab index of row gi for H (one cell lower on the minus side) and E.
@@ -663,6 +678,39 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const float* pin = a.psi_in;
     const size_t xs = xslab_floats(d), ys = yslab_floats(d);
 
+    // x-slab coefficients of each owned row (RA - 1, RB, RE, RF for H and E):
+    // row-uniform, so they sit in scalar registers for the whole launch.
+    auto uni = [](float v) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    };
+    // (The adjoint, which holds more registers, reads them from an LDS table.)
+    __shared__ float sP[ADJ ? 8 * kPmax : 1];
+    if (ADJ)
+        for (int e = threadIdx.x; e < 8 * P; e += NT) sP[e] = d.pml[e];
+    float xH[CPT][4], xE[CPT][4];
+#pragma unroll
+    for (int m = 0; m < CPT && !ADJ; ++m) {
+        const int gi = gi0 + r0 + m;
+        const bool in = P > 0 && gi >= 0 && gi < NI;
+        const int qh = in ? xq_h(gi) : -1;
+        const int qe = in ? xq_e(gi) : -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            xH[m][c] = qh >= 0 ? uni(d.pml[(4 + c) * P + qh]) : 0.f;
+            xE[m][c] = qe >= 0 ? uni(d.pml[c * P + qe]) : 0.f;
+        }
+        xH[m][0] = xH[m][0] - 1.0f;
+        xE[m][0] = xE[m][0] - 1.0f;
+    }
+    auto XH = [&](int m, int q, int c) -> float {
+        if constexpr (ADJ) return c == 0 ? sP[4 * P + q] - 1.0f : sP[(4 + c) * P + q];
+        else return xH[m][c];
+    };
+    auto XE = [&](int m, int q, int c) -> float {
+        if constexpr (ADJ) return c == 0 ? sP[q] - 1.0f : sP[c * P + q];
+        else return xE[m][c];
+    };
+
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int r = r0 + m, gi = gi0 + r;
@@ -672,9 +720,6 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         ez[m] = ex ? a.Ez_in[g] : 0.f;
         hx[m] = ex ? a.Hx_in[g] : 0.f;
         hy[m] = ex ? a.Hy_in[g] : 0.f;
-        sEz[o] = ez[m];
-        sHx[o] = hx[m];
-        sHy[o] = hy[m];
         const size_t c = (size_t)gi * pitch + gj;
         cE0[m] = ex ? d.coef[c] : 0.f;
         cE1[m] = ex ? d.coef[pl + c] : 0.f;
@@ -685,158 +730,269 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (ac) act |= 1u << m;
         const int qh = (ac && P > 0) ? xq_h(gi) : -1;
         const int qe = (ac && P > 0) ? xq_e(gi) : -1;
-        pHx[m] = qh >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] : 0.f;
-        pEx[m] = qe >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] : 0.f;
-        pHy[m] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(ykh - 4) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
-        pEy[m] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(yke - 4) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
+        sPsi[0][o] = qh >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] : 0.f;
+        sPsi[1][o] = qe >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] : 0.f;
+        sPsi[2][o] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
+        sPsi[3][o] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
     }
-    // Receiver list of this (shot, tile): local cell packed as li*64+lj, plus
-    // the receiver number (forward: centre receivers; adjoint: region ones).
+    sEzTop[w * 64 + lane] = ez[0];
+    sHyBot[w * 64 + lane] = hy[CPT - 1];
+
+    // This wave's receivers (forward: tile-centre ones; adjoint: region
+    // ones), compacted in list order with a ballot per 64 entries.
     const int lst = a.rx_ptr ? a.rx_ptr[s * ntiles + tile] : 0;
     const int lcount = (a.rx_ptr ? a.rx_ptr[s * ntiles + tile + 1] : 0) - lst;
-    for (int e = threadIdx.x; e < lcount && e < kMaxList / 2; e += NT) {
-        const int t = a.rx_idx[lst + e];
-        const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
-        const bool ok = li >= 0 && li < RI && lj >= 0 && lj < 64;
-        sList[2 * e] = ok ? li * 64 + lj : -1;
-        sList[2 * e + 1] = t - (t / d.nrx) * d.nrx;
+    int wn = 0;
+    for (int base = 0; base < lcount; base += 64) {
+        const int e = base + lane;
+        int ent = -1;
+        if (e < lcount) {
+            const int t = a.rx_idx[lst + e];
+            const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
+            if (li >= r0 && li < r0 + CPT && lj >= 0 && lj < 64)
+                ent = ((li - r0) << 26) | (lj << 20) | (t - (t / d.nrx) * d.nrx);
+        }
+        const unsigned long long bal = __ballot(ent >= 0);
+        const int pos = wn + (int)__popcll(bal & ((1ull << lane) - 1ull));
+        if (ent >= 0 && pos < kWaveList) sRx[w][pos] = ent;
+        wn += (int)__popcll(bal);
     }
-    const int nlist = lcount < kMaxList / 2 ? lcount : kMaxList / 2;
+    wn = wn < kWaveList ? wn : kWaveList;
+    // one entry per lane, kWaveList / 64 registers
+    int rxent[kWaveList / 64];
+#pragma unroll
+    for (int c = 0; c < kWaveList / 64; ++c) rxent[c] = (c * 64 + lane < wn) ? sRx[w][c * 64 + lane] : 0;
+
+    // Cells of this thread holding a source (forward).
+    unsigned srcbits = 0u;
+    if (!ADJ) {
+        for (int kk = 0; kk < d.nsrc; ++kk) {
+            const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
+            const int li = sc[0] - gi0 - r0;
+            if (sc[1] == gj && li >= 0 && li < CPT) srcbits |= 1u << li;
+        }
+    }
+
     const float dx = d.dx;
     double ae[CPT], as_[CPT];
-    float fnext[CPT];
 #pragma unroll
-    for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; fnext[m] = 0.f; }
+    for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; }
     __syncthreads();
 
-    // H half-step on the owned cells; publishes Hx (read by lane+1's E
-    // update) and the block's last Hy row (read by the next block's first row).
+    // H half-step of the owned cells.
     auto h_half = [&]() {
+        const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
+            const float e = ez[m];
+            const float ezj = lane_next(e);
+            const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below;
             if (!(act & (1u << m))) continue;
             const int r = r0 + m, gi = gi0 + r;
             const int o = r * 64 + lane;
-            const float e = ez[m];
-            const float ezj = sEz[o + 1];
-            const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ((r + 1 < RI) ? sEz[o + 64] : 0.f);
             if (gi >= 1 || yqh >= 0) {
                 float v = hx[m];
                 if (gi >= 1) v = v - cH1[m] * (ezj - e);
                 if (yqh >= 0) {
+                    const float c4 = cH4[m];
                     const float dd = (ezj - e) / dx;
-                    v = v - cH4[m] * (yh[0] * dd + yh[1] * pHy[m]);
-                    pHy[m] = yh[2] * pHy[m] - yh[3] * dd;
+                    const float ph = sPsi[2][o];
+                    v = v - c4 * (yh[0] * dd + yh[1] * ph);
+                    sPsi[2][o] = yh[2] * ph - yh[3] * dd;
                 }
                 hx[m] = v;
             }
-            if (gj >= 1 || (P > 0 && xq_h(gi) >= 0)) {
+            const int q = P > 0 ? xq_h(gi) : -1;
+            if (gj >= 1 || q >= 0) {
                 float v = hy[m];
                 if (gj >= 1) v = v + cH1[m] * (ezi - e);
-                const int q = P > 0 ? xq_h(gi) : -1;
                 if (q >= 0) {
+                    const float c4 = cH4[m];
                     const float dd = (ezi - e) / dx;
-                    v = v + cH4[m] * ((sP[4 * P + q] - 1.0f) * dd + sP[5 * P + q] * pHx[m]);
-                    pHx[m] = sP[6 * P + q] * pHx[m] - sP[7 * P + q] * dd;
+                    const float ph = sPsi[0][o];
+                    v = v + c4 * (XH(m, q, 0) * dd + XH(m, q, 1) * ph);
+                    sPsi[0][o] = XH(m, q, 2) * ph - XH(m, q, 3) * dd;
                 }
                 hy[m] = v;
             }
-            sHx[o] = hx[m];
-            if (m == CPT - 1) sHy[o] = hy[m];
         }
+        sHyBot[w * 64 + lane] = hy[CPT - 1];
     };
 
     // E update of owned cell m (main + CPML in slab order x0, y0, xmax, ymax).
-    auto e_cell = [&](int m, int r, int gi, int o) -> float {
+    auto e_cell = [&](int m, int gi, int o, float hyim, float hxjm) -> float {
         float v = ez[m];
-        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1 && r >= 1) ? sHy[o - 64] : 0.f);
-        const float hxjm = (gj >= 1 && lane >= 1) ? sHx[o - 1] : 0.f;
         if (gi >= 1 && gj >= 1) v = (cE0[m] * v + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
         const int q = P > 0 ? xq_e(gi) : -1;
-        const bool xminus = gi <= P;
-        float dxd = 0.f, dyd = 0.f;
-        if (q >= 0) dxd = (hy[m] - hyim) / dx;
-        if (yqe >= 0) dyd = (hx[m] - hxjm) / dx;
-        if (q >= 0 && xminus) {
-            v = v + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
-            pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
-        }
-        if (yqe >= 0 && yke == EY0) {
-            v = v - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
-            pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
-        }
-        if (q >= 0 && !xminus) {
-            v = v + cE4[m] * ((sP[q] - 1.0f) * dxd + sP[P + q] * pEx[m]);
-            pEx[m] = sP[2 * P + q] * pEx[m] - sP[3 * P + q] * dxd;
-        }
-        if (yqe >= 0 && yke == EYM) {
-            v = v - cE4[m] * (ye[0] * dyd + ye[1] * pEy[m]);
-            pEy[m] = ye[2] * pEy[m] - ye[3] * dyd;
+        if (q >= 0 || yqe >= 0) {
+            const float c4 = cE4[m];
+            const bool xminus = gi <= P;
+            float dxd = 0.f, dyd = 0.f;
+            if (q >= 0) dxd = (hy[m] - hyim) / dx;
+            if (yqe >= 0) dyd = (hx[m] - hxjm) / dx;
+            if (q >= 0 && xminus) {
+                const float p = sPsi[1][o];
+                v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
+                sPsi[1][o] = XE(m, q, 2) * p - XE(m, q, 3) * dxd;
+            }
+            if (yqe >= 0 && ye_lo) {
+                const float p = sPsi[3][o];
+                v = v - c4 * (ye[0] * dyd + ye[1] * p);
+                sPsi[3][o] = ye[2] * p - ye[3] * dyd;
+            }
+            if (q >= 0 && !xminus) {
+                const float p = sPsi[1][o];
+                v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
+                sPsi[1][o] = XE(m, q, 2) * p - XE(m, q, 3) * dxd;
+            }
+            if (yqe >= 0 && !ye_lo) {
+                const float p = sPsi[3][o];
+                v = v - c4 * (ye[0] * dyd + ye[1] * p);
+                sPsi[3][o] = ye[2] * p - ye[3] * dyd;
+            }
         }
         return v;
     };
 
-    for (int k = 0; k < nsteps; ++k) {
-        const int n = ADJ ? n0 - k : n0 + k;
-        if (!ADJ) {
-            // receivers of this tile's centre, sampled before the step
-            for (int e = threadIdx.x; e < nlist; e += NT) {
-                const int cell = sList[2 * e];
-                if (cell >= 0)
-                    d.traces[((size_t)s * d.nt + n) * d.nrx + sList[2 * e + 1]] = sEz[cell];
-            }
-            h_half();
-            __syncthreads();
+    // Interior regions (no CPML cell, no domain edge, every cell updated)
+    // take a branch-free copy of the step loop.
+    const bool interior = gi0 >= P + 1 && gi0 + RI <= NI - P && gj0 >= P + 1 && gj0 + 64 <= NJ - P;
+
+    auto h_fast = [&]() {
+        const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
 #pragma unroll
-            for (int m = 0; m < CPT; ++m) {
-                if (!(act & (1u << m))) continue;
-                const int r = r0 + m, gi = gi0 + r;
-                const int o = r * 64 + lane;
-                float v = e_cell(m, r, gi, o);
-                for (int kk = 0; kk < d.nsrc; ++kk) {
-                    const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
-                    if (sc[0] == gi && sc[1] == gj) v = v - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
-                }
-                ez[m] = v;
-                sEz[o] = v;
-                if (d.snaps && r >= K && r < K + TI && centre_col)
-                    d.snaps[snap_off(d, n, s, gi, gj)] = v;
-            }
-            __syncthreads();
-        } else {
-            // adjoint source (back.cu:24-37) at receivers inside the region,
-            // applied by the owner of the cell in list order
-            for (int e = 0; e < nlist; ++e) {
-                const int cell = sList[2 * e];
-                const int li = cell >> 6, lj = cell & 63;
-                if (cell >= 0 && lj == lane && li >= r0 && li < r0 + CPT) {
-                    const float g = d.resid[((size_t)s * d.nt + n) * d.nrx + sList[2 * e + 1]];
-#pragma unroll
-                    for (int m = 0; m < CPT; ++m)
-                        if (li == r0 + m) ez[m] = ez[m] - g;
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < CPT; ++m) {
-                if (!(act & (1u << m))) continue;
-                const int r = r0 + m, gi = gi0 + r;
-                const int o = r * 64 + lane;
-                const float lam = e_cell(m, r, gi, o);
-                ez[m] = lam;
-                sEz[o] = lam;
-                if (r >= K && r < K + TI && centre_col) {
-                    const float fn = (k == 0) ? d.snaps[snap_off(d, n, s, gi, gj)] : fnext[m];
-                    const float fp = d.snaps[snap_off(d, n - 1, s, gi, gj)];
-                    ae[m] += (double)((fn - fp) * lam);
-                    as_[m] += (double)(fn * lam);
-                    fnext[m] = fp;
-                }
-            }
-            __syncthreads();
-            h_half();
-            __syncthreads();
+        for (int m = 0; m < CPT; ++m) {
+            const float e = ez[m];
+            const float ezj = lane_next(e);
+            const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below;
+            hx[m] = hx[m] - cH1[m] * (ezj - e);
+            hy[m] = hy[m] + cH1[m] * (ezi - e);
         }
+        sHyBot[w * 64 + lane] = hy[CPT - 1];
+    };
+
+    // Snapshot of (step n, this thread's row m): snap_thr + slot(n) * snap_slot + m * pitch.
+    const int64_t snap_slot = (int64_t)d.S * NI * pitch;
+    float* const snap_thr = d.snaps ? d.snaps + ((int64_t)s * NI + gi0 + r0) * pitch + gj : nullptr;
+    auto snap_at = [&](int n, int m) -> float* {
+        const int slot = d.snap_mod ? n % d.snap_mod : n - d.snap_base;
+        return snap_thr + slot * snap_slot + (int64_t)m * pitch;
+    };
+
+    // adjoint: snapshots of steps n (fcur) and n - 1 (fprv) of the current step
+    float fcur[CPT], fprv[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int r = r0 + m;
+        const bool g = ADJ && nsteps > 0 && (act & (1u << m)) && r >= K && r < K + TI && centre_col;
+        fcur[m] = g ? *snap_at(n0, m) : 0.f;
+        fprv[m] = g ? *snap_at(n0 - 1, m) : 0.f;
     }
+
+    auto steps = [&](auto fast_tag) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        for (int k = 0; k < nsteps; ++k) {
+            const int n = ADJ ? n0 - k : n0 + k;
+            if (!ADJ) {
+                // receivers of this wave, sampled before the step
+#pragma unroll
+                for (int c = 0; c < kWaveList / 64; ++c) {
+                    const int cnt = wn - c * 64 < 64 ? wn - c * 64 : 64;
+                    for (int q = 0; q < cnt; ++q) {
+                        const int ent = __builtin_amdgcn_readlane(rxent[c], q);
+                        if (lane == ((ent >> 20) & 63)) {
+                            const int me = ent >> 26;
+                            float v = 0.f;
+#pragma unroll
+                            for (int m = 0; m < CPT; ++m) v = (m == me) ? ez[m] : v;
+                            d.traces[((size_t)s * d.nt + n) * d.nrx + (ent & 0xFFFFF)] = v;
+                        }
+                    }
+                }
+                if (FAST) h_fast(); else h_half();
+                __syncthreads();
+                const float hy_above = (w > 0) ? sHyBot[(w - 1) * 64 + lane] : 0.f;
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const float hxjm = lane_prev(hx[m]);
+                    const int r = r0 + m, gi = gi0 + r;
+                    float v;
+                    if (FAST) {
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
+                        v = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+                    } else {
+                        if (!(act & (1u << m))) continue;
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1) ? hy_above : 0.f);
+                        v = e_cell(m, gi, r * 64 + lane, hyim, (gj >= 1) ? hxjm : 0.f);
+                    }
+                    if (srcbits & (1u << m)) {
+                        for (int kk = 0; kk < d.nsrc; ++kk) {
+                            const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
+                            if (sc[0] == gi && sc[1] == gj) v = v - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
+                        }
+                    }
+                    ez[m] = v;
+                    if (snap_thr && r >= K && r < K + TI && centre_col) *snap_at(n, m) = v;
+                }
+                sEzTop[w * 64 + lane] = ez[0];
+                __syncthreads();
+            } else {
+                // adjoint source (back.cu:24-37) at receivers inside the region,
+                // applied by the owner of the cell in list order
+#pragma unroll
+                for (int c = 0; c < kWaveList / 64; ++c) {
+                    const int cnt = wn - c * 64 < 64 ? wn - c * 64 : 64;
+                    const int mine = rxent[c];
+                    const float gv = lane < cnt ? d.resid[((size_t)s * d.nt + n) * d.nrx + (mine & 0xFFFFF)] : 0.f;
+                    for (int q = 0; q < cnt; ++q) {
+                        const int ent = __builtin_amdgcn_readlane(mine, q);
+                        const float g = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gv), q));
+                        // x - 0 == x, so every row takes the subtraction and
+                        // only the receiver's row changes (keeps ez[] in registers)
+                        const int me = (lane == ((ent >> 20) & 63)) ? (ent >> 26) : -1;
+#pragma unroll
+                        for (int m = 0; m < CPT; ++m) ez[m] = ez[m] - ((m == me) ? g : 0.f);
+                    }
+                }
+                const float hy_above = (w > 0) ? sHyBot[(w - 1) * 64 + lane] : 0.f;
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const float hxjm = lane_prev(hx[m]);
+                    const int r = r0 + m, gi = gi0 + r;
+                    float lam;
+                    if (FAST) {
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
+                        lam = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+                    } else {
+                        if (!(act & (1u << m))) continue;
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1) ? hy_above : 0.f);
+                        lam = e_cell(m, gi, r * 64 + lane, hyim, (gj >= 1) ? hxjm : 0.f);
+                    }
+                    ez[m] = lam;
+                    if (r >= K && r < K + TI && centre_col) {
+                        const float fn = fcur[m];
+                        const float fp = fprv[m];
+                        ae[m] += (double)((fn - fp) * lam);
+                        as_[m] += (double)(fn * lam);
+                    }
+                }
+                // snapshots for the next step, loaded one step ahead
+#pragma unroll
+                for (int m = 0; m < CPT; ++m) {
+                    const int r = r0 + m;
+                    fcur[m] = fprv[m];
+                    if (k + 1 < nsteps && (FAST || (act & (1u << m))) && r >= K && r < K + TI && centre_col)
+                        fprv[m] = *snap_at(n - 2, m);
+                }
+                sEzTop[w * 64 + lane] = ez[0];
+                __syncthreads();
+                if (FAST) h_fast(); else h_half();
+                __syncthreads();
+            }
+        }
+    };
+    if (interior) steps(std::true_type{});
+    else steps(std::false_type{});
 
     // ---- write back the tile centre ----
     float* pout = a.psi_out;
@@ -850,12 +1006,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         a.Hx_out[g] = hx[m];
         a.Hy_out[g] = hy[m];
         if (!(act & (1u << m))) continue;
+        const int o = r * 64 + lane;
         const int qh = P > 0 ? xq_h(gi) : -1;
         const int qe = P > 0 ? xq_e(gi) : -1;
-        if (qh >= 0) pout[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] = pHx[m];
-        if (qe >= 0) pout[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] = pEx[m];
-        if (yqh >= 0) pout[4 * xs + (size_t)(ykh - 4) * ys + ((size_t)s * NI + gi) * P + yqh] = pHy[m];
-        if (yqe >= 0) pout[4 * xs + (size_t)(yke - 4) * ys + ((size_t)s * NI + gi) * P + yqe] = pEy[m];
+        if (qh >= 0) pout[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] = sPsi[0][o];
+        if (qe >= 0) pout[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] = sPsi[1][o];
+        if (yqh >= 0) pout[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] = sPsi[2][o];
+        if (yqe >= 0) pout[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] = sPsi[3][o];
         if (ADJ) {
             double* g0 = a.gshot + ((size_t)s * NI + gi) * NJ + gj;
             double* g1 = g0 + (size_t)d.S * NI * NJ;
@@ -984,7 +1141,8 @@ using TBMultiKernel = void (*)(const unsigned char*, int);
 TBMultiKernel tb_multi_kernel(int ri, int nt, int k, bool adj) {
     FGPR_TBM_CASE(32, 512, 2) FGPR_TBM_CASE(32, 512, 4) FGPR_TBM_CASE(32, 512, 6)
     FGPR_TBM_CASE(64, 1024, 2) FGPR_TBM_CASE(64, 1024, 4) FGPR_TBM_CASE(64, 1024, 6)
-    FGPR_TBM_CASE(64, 512, 4) FGPR_TBM_CASE(64, 512, 6)
+    FGPR_TBM_CASE(64, 512, 2) FGPR_TBM_CASE(64, 512, 4) FGPR_TBM_CASE(64, 512, 6)
+    FGPR_TBM_CASE(64, 512, 8)
     return nullptr;
 }
 #undef FGPR_TBM_CASE
@@ -1000,6 +1158,7 @@ int tb_check(const fgpr_problem& p, bool adj) {
     if (!p.Ez2 || !p.Hx2 || !p.Hy2 || (p.P > 0 && !p.psi2))
         return fail(FGPR_EINVAL, "time-blocked path needs the ping-pong buffers");
     if (p.P > kPmax) return fail(FGPR_EINVAL, "CPML thicker than 128 cells");
+    if (p.nrx >= (1 << 20)) return fail(FGPR_EINVAL, "time-blocked path holds at most 2^20 - 1 receivers per shot");
     if (adj && !p.gshot) return fail(FGPR_EINVAL, "time-blocked adjoint needs gshot");
     if ((adj ? !p.tb_inj_ptr : !p.tb_rx_ptr) && p.nrx > 0)
         return fail(FGPR_EINVAL, "time-blocked path needs the per-tile receiver tables");

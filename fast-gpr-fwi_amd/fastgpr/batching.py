@@ -29,7 +29,7 @@ from .checkpoint import States
 from ._native import check, lib
 
 MULTI_VARIANTS = [(32, 512, 2), (32, 512, 4), (32, 512, 6), (64, 1024, 2), (64, 1024, 4),
-                  (64, 1024, 6), (64, 512, 4), (64, 512, 6)]
+                  (64, 1024, 6), (64, 512, 2), (64, 512, 4), (64, 512, 6), (64, 512, 8)]
 
 
 def job_bytes(NI, NJ, S, nt, nrx, P, C, gradient=True) -> int:
