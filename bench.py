@@ -40,7 +40,7 @@ def parse():
                     help="'auto' (autotuned), 'none' (per-step kernels) or rows,threads,steps")
     ap.add_argument("--plain", action="store_true",
                     help="also time the plain one-shot-per-launch eager path")
-    ap.add_argument("--cpu-steps", type=int, default=400,
+    ap.add_argument("--cpu-steps", type=int, default=3000,
                     help="time steps of the CPU oracle sample (forward + adjoint)")
     ap.add_argument("--profile-reps", type=int, default=200)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -333,9 +333,12 @@ def plain_path(wl, er, se, mr, obs, src, rx, cu):
 
 
 def cpu_baseline(wl, nsteps: int):
-    """Oracle FP64 forward + adjoint over the first ``nsteps`` time steps of
-    the same batch (all shots), on one host core."""
+    """The FP64 oracle's forward + adjoint over the first ``nsteps`` time
+    steps of the same batch (all shots): the C/OpenMP restatement on every
+    host thread OpenMP is given, or the numpy restatement on one core when
+    the C library is not built."""
     import numpy as np
+    from oracle import c_oracle as CO
     from oracle import fdtd_oracle as O
     sys.path.insert(0, str(REPO / "tests"))
     from helpers import oracle_problem
@@ -343,16 +346,26 @@ def cpu_baseline(wl, nsteps: int):
     pb.nt = nsteps
     pb.wavelet = pb.wavelet[:, :nsteps]
     resid = np.random.default_rng(0).standard_normal((wl.S, nsteps, wl.rx.shape[1]))
-    t0 = time.perf_counter()
-    tr, snaps = O.run_forward(pb, np.float64, store=True)
-    O.adjoint_reference(*O.setup(pb, np.float64, adjoint=True), pb.P, pb.dx, pb.rx, resid, snaps)
+    c, pml = O.setup(pb, np.float64)
+    cadj, _ = O.setup(pb, np.float64, adjoint=True)
+    scale = O.source_scale(pb.wavelet, pb.dx, np.float64)
+    if CO.available():
+        impl, cores = "C/OpenMP FP64 oracle", CO.threads()
+        fwd, adj = CO.forward, CO.adjoint_reference
+        t0 = time.perf_counter()
+        tr, snaps = fwd(c, pml, pb.P, pb.dx, pb.src, scale, pb.rx, nsteps, True)
+    else:
+        impl, cores = "numpy FP64 oracle", 1
+        adj = O.adjoint_reference
+        t0 = time.perf_counter()
+        tr, snaps = O.forward(c, pml, pb.P, pb.dx, pb.src, scale, pb.rx, nsteps, np.float64, True)
+    adj(cadj, pml, pb.P, pb.dx, pb.rx, resid, snaps)
     el = time.perf_counter() - t0
     cu = 2 * wl.S * wl.NI * wl.NJ * nsteps
-    cores = 1
     return {"value": round(cu / el / 1e9, 5), "unit": "Gcell-updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"numpy FP64 oracle, {wl.name}, all {wl.S} shots, first {nsteps} of "
-                      f"{wl.nt} steps forward + adjoint ({el:.1f} s)"}
+            "sample": f"{impl}, {wl.name}, all {wl.S} shots, first {nsteps} of "
+                      f"{wl.nt} steps forward + adjoint ({el:.1f} s, {cores} threads)"}
 
 
 if __name__ == "__main__":
