@@ -46,6 +46,13 @@ def parse():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch a 256 MiB copy and the dominant kernels a few times "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--uneven", action="store_true",
+                    help="time the uneven batch (4 models, 64 shots) through the multi-problem "
+                         "scheduler against one model at a time; prints its own JSON line")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="run the serving loop under Poisson arrivals (20 requests/s) for this "
+                         "many seconds of arrivals; prints its own JSON line")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (requests/s)")
     ap.add_argument("--traffic", default=str(Path(__file__).resolve().parent / "profiles" /
                                              "traffic_c2_crosshole.json"),
                     help="measured HBM bytes per launch (from the counter runs), if present")
@@ -104,6 +111,12 @@ def main():
     wl = c2_crosshole()
     if args.pmc_probe:
         pmc_probe(wl, dev, args)
+        return
+    if args.uneven:
+        uneven_bench(dev)
+        return
+    if args.serve > 0:
+        serve_bench(dev, args.serve, args.rate)
         return
     t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
     er_true, se_true, mr = t(wl.er), t(wl.se), t(wl.mr)
@@ -330,6 +343,175 @@ def plain_path(wl, er, se, mr, obs, src, rx, cu):
     return {"value": round(cu / el / 1e9, 3), "unit": "Gcell-updates/s",
             "ms_per_gradient": round(el * 1e3, 1),
             "what": "per-step eager launches, one shot per launch sequence, no graph"}
+
+
+def _job_for(wl, dev, tb, checkpoint):
+    """A gradient job of workload ``wl`` (initial model, zero observed data),
+    checkpointed when ``checkpoint`` is true."""
+    import math
+    import numpy as np
+    import torch
+    import fastgpr
+    from fastgpr.batching import Job
+    t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
+    er, se, mr = torch.clamp(t(wl.er0), min=1), torch.clamp(t(wl.se0), min=0), t(wl.mr)
+    G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                               None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                               er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
+                               False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
+    C = None
+    if checkpoint:
+        K = tb[2]
+        c = int(math.ceil(math.sqrt(3.0 * wl.nt)))
+        C = ((c + 2 * K - 1) // (2 * K)) * 2 * K
+    obs = torch.zeros((wl.S, wl.nt, wl.rx.shape[1]), dtype=torch.float32, device=dev)
+    return Job(G, er, se, mr, obs=obs, tb=tb, checkpoint=C), C
+
+
+def uneven_bench(dev):
+    """Uneven batch: 4 models of different sizes and step counts, 64 shots,
+    advanced together by multi-problem launches (one per direction per tick),
+    against the same gradients run one model at a time through ``compute``
+    (time-blocked, captured graphs).  Checkpoint recompute is not counted."""
+    import numpy as np
+    import torch
+    import fastgpr
+    from fastgpr.batching import Scheduler
+    from fastgpr.workloads import c3_uneven
+    wls = c3_uneven()
+    tb = (64, 1024, 6)
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info(dev)
+    # Checkpoint what compute(checkpoint="auto") checkpoints (a history over
+    # half the free memory), then the largest remaining ones until all fit.
+    from fastgpr.batching import job_bytes
+    full = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, w.nt) for w in wls]
+    ck = [b > 0.5 * free for b in full]
+    small = lambda k: job_bytes(wls[k].NI, wls[k].NJ, wls[k].S, wls[k].nt, wls[k].rx.shape[1],
+                                wls[k].P, 144)
+    while sum(small(k) if ck[k] else full[k] for k in range(len(wls))) > 0.8 * free:
+        k = max((k for k in range(len(wls)) if not ck[k]), key=lambda k: full[k])
+        ck[k] = True
+    cu = sum(w.cell_updates_per_gradient() for w in wls)
+
+    def batched():
+        jobs = [_job_for(w, dev, tb, c) for w, c in zip(wls, ck)]
+        sch = Scheduler(dev)
+        for j, _ in jobs:
+            sch.add(j)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sch.run()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sch.launches, [c for _, c in jobs]
+
+    batched()                                    # warm-up (graphs, allocator)
+    el_b, launches, ckpts = batched()
+
+    def one_at_a_time():
+        t0 = time.perf_counter()
+        for w in wls:
+            t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
+            er = torch.clamp(t(w.er0), min=1).requires_grad_(True)
+            se = torch.clamp(t(w.se0), min=0).requires_grad_(True)
+            pred, _ = fastgpr.compute(dev, w.dx, w.NJ, w.NI, 1, ['hertzian', 'z'],
+                                      ['ricker', 1, w.freq], w.S, er, se, t(w.mr), 1,
+                                      w.rx.shape[1], time_windows=w.time_window, dt=w.dt,
+                                      pmlthick=w.P, tv=False, customsource=torch.tensor(w.src),
+                                      customreceiver=torch.tensor(w.rx), time_block=tb,
+                                      checkpoint="auto")
+            (0.5 * (pred ** 2).sum()).backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    one_at_a_time()
+    el_s = one_at_a_time()
+    print(json.dumps({
+        "metric": "uneven batch gradient Gcell-updates/s (fwd+adjoint)",
+        "value": round(cu / el_b / 1e9, 3), "unit": "Gcell-updates/s",
+        "seconds": round(el_b, 3), "launches": launches,
+        "one_model_at_a_time": {"value": round(cu / el_s / 1e9, 3), "seconds": round(el_s, 3)},
+        "speedup": round(el_s / el_b, 3),
+        "config": {"workload": "4 models, 64 shots: " + ", ".join(w.name + f"/{w.nt}" for w in wls),
+                   "cell_updates": cu, "time_block": list(tb),
+                   "checkpoint_interval": ckpts}}), flush=True)
+
+
+def serve_bench(dev, seconds, rate):
+    """Serving loop under Poisson arrivals: requests (grid uniform over
+    128x128, 256x128, 512x256, 1024x512; 1-32 shots; 1500-5000 steps by grid
+    diagonal) are generated beforehand, submitted from a thread at their
+    arrival times, admitted by device bytes and continuously batched.  Reports
+    p50/p99 latency, sustained throughput and the largest difference between a
+    request served in the mix and the same request served alone (8 sampled)."""
+    import threading
+    import numpy as np
+    import torch
+    from fastgpr.serving import Request, Server, poisson_trace, serving_request
+    rng = np.random.default_rng(4)
+    arrivals = poisson_trace(rate, seconds, seed=4)
+    reqs = [serving_request(i, rng) for i in range(len(arrivals))]
+    offered = sum(r.wl.cell_updates_per_gradient() for r in reqs)
+    # warm-up: one small request of every grid (graph and allocator set-up)
+    warm = Server(dev)
+    for g in ((128, 128), (256, 128), (512, 256), (1024, 512)):
+        warm.submit(serving_request(-1, np.random.default_rng(5), grids=(g,), shots=(2, 2),
+                                    steps=(60, 60)))
+    warm.run()
+    torch.cuda.synchronize()
+
+    del warm
+    torch.cuda.empty_cache()
+    srv = Server(dev)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for at, r in zip(arrivals, reqs):
+            d = t0 + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(r)
+
+    th = threading.Thread(target=submitter, daemon=True)
+    th.start()
+    last = time.perf_counter()
+    while True:
+        busy = srv.step()
+        if not busy:
+            if not th.is_alive() and srv.pending.empty():
+                break
+            time.sleep(0.0005)
+        now = time.perf_counter()
+        if now - last > 20:
+            print(f"serving: {srv.stats.completed}/{len(reqs)} done, {len(srv.active)} active, "
+                  f"{len(srv.waiting)} waiting", file=sys.stderr, flush=True)
+            last = now
+    torch.cuda.synchronize()
+    srv.stats.finished = time.perf_counter()
+    summ = srv.stats.summary()
+    # alone vs in-mix on a sample of the completed requests
+    worst = 0.0
+    done = [r for r in reqs if r.result and "er" in r.result]
+    for r in done[:: max(1, len(done) // 8)][:8]:
+        a = Server(dev)
+        ra = a.submit(Request(r.rid, r.wl, r.model, r.obs, tb=r.tb))
+        a.run()
+        torch.cuda.synchronize()
+        for k in ("er", "se", "traces"):
+            x, y = r.result[k].double(), ra.result[k].double()
+            worst = max(worst, float((x - y).abs().max() / max(y.abs().max(), 1e-30)))
+    print(json.dumps({
+        "metric": "serving: p50/p99 request latency, sustained gradient Gcell-updates/s",
+        "p50_latency_s": round(summ["p50_latency_s"], 3),
+        "p99_latency_s": round(summ["p99_latency_s"], 3),
+        "sustained_gcell_updates_per_s": round(summ["sustained_gcell_updates_per_s"], 3),
+        "completed": summ["completed"], "requests": len(reqs),
+        "arrival_window_s": seconds, "rate_per_s": rate, "wall_s": round(summ["wall_s"], 2),
+        "offered_gcell_updates_per_s": round(offered / seconds / 1e9, 3),
+        "max_concurrent_requests": summ["max_concurrent_requests"],
+        "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
+        "launches": summ["launches"],
+        "alone_vs_mix_max_rel_diff": worst}), flush=True)
 
 
 def cpu_baseline(wl, nsteps: int):
