@@ -1289,6 +1289,189 @@ int check_problem(const fgpr_problem* p) {
     return FGPR_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Total-variation denoiser (src/tv.py:135-217): ADMM on the max-normalised
+// image, zero-padded by one cell, periodic differences on the padded grid,
+// soft-thresholded split variables.  All iterations run on the device as a
+// sequence of launches; the stopping test ||X - X_prev|| < tol is evaluated
+// on the device (fixed-order FP64 sum of per-block partials), and every later
+// launch returns at once.  The float32
+// operation order follows the reference expressions; the 4-neighbour sum
+// (conv2d in the reference, whose summation order is the library's) is
+// (up + down) + (left + right).
+struct TVWork {
+    float* X[2];      // ping-pong iterates, (M+2) x (N+2)
+    float* Y;         // normalised data, zero border
+    float* Zx; float* Zy; float* Ux; float* Uy;
+    float* scal;      // [0] data max
+    int* flags;       // [0] done, [1] iteration that converged
+    double* part;     // one partial sum of squares per X-sweep block
+    int M2, N2;
+};
+
+constexpr int kTVBx = 64, kTVBy = 4;
+
+__global__ void __launch_bounds__(1024) k_tv_max(const float* __restrict__ data, int64_t n, TVWork w) {
+    __shared__ float red[1024];
+    float m = -INFINITY;
+    for (int64_t k = threadIdx.x; k < n; k += 1024) m = fmaxf(m, data[k]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        w.scal[0] = red[0];
+        w.flags[0] = 0;
+        w.flags[1] = -1;
+    }
+}
+
+__global__ void __launch_bounds__(kTVBx * kTVBy) k_tv_init(const float* __restrict__ data, TVWork w) {
+    const int j = blockIdx.x * kTVBx + threadIdx.x, i = blockIdx.y * kTVBy + threadIdx.y;
+    if (i >= w.M2 || j >= w.N2) return;
+    const size_t c = (size_t)i * w.N2 + j;
+    const int N = w.N2 - 2;
+    const bool in = i >= 1 && i < w.M2 - 1 && j >= 1 && j < w.N2 - 1;
+    const float den = w.scal[0] + 1e-6f;
+    const float y = in ? data[(size_t)(i - 1) * N + (j - 1)] / den : 0.f;
+    w.X[0][c] = y; w.X[1][c] = y; w.Y[c] = y;
+    w.Zx[c] = 0.f; w.Zy[c] = 0.f; w.Ux[c] = 0.f; w.Uy[c] = 0.f;
+}
+
+// One interior cell of the X sweep (Jacobi from Xo into Xn); returns the
+// squared change.
+__device__ __forceinline__ double tv_x_cell(const TVWork& w, const float* __restrict__ Xo,
+                                            float* __restrict__ Xn, int i, int j, float lam,
+                                            float lr, float den, float rho) {
+    const int N2 = w.N2;
+    const size_t c = (size_t)i * N2 + j;
+    // Dxt Z = Z - Z[j+1], Dyt Z = Z - Z[i+1] (no wrap inside the interior)
+    const float dxz = w.Zx[c] - w.Zx[c + 1], dyz = w.Zy[c] - w.Zy[c + N2];
+    const float dxu = w.Ux[c] - w.Ux[c + 1], dyu = w.Uy[c] - w.Uy[c + N2];
+    const float rhs = (w.Y[c] + lr * (dxz + dyz)) - lam * (dxu + dyu);
+    const float nb = (Xo[c - N2] + Xo[c + N2]) + (Xo[c - 1] + Xo[c + 1]);
+    const float x = ((nb * lam) * rho + rhs) / den;
+    Xn[c] = x;
+    const double e = (double)x - (double)Xo[c];
+    return e * e;
+}
+
+// One cell of the split-variable and multiplier update from X (periodic
+// backward differences over the padded grid).
+__device__ __forceinline__ void tv_zu_cell(const TVWork& w, const float* __restrict__ X, int i,
+                                           int j, float rho, float inv_rho) {
+    const int M2 = w.M2, N2 = w.N2;
+    const size_t c = (size_t)i * N2 + j;
+    const float x = X[c];
+    const float dxx = x - X[(size_t)i * N2 + (j > 0 ? j - 1 : N2 - 1)];
+    const float dyx = x - X[(size_t)(i > 0 ? i - 1 : M2 - 1) * N2 + j];
+    const float tx = (w.Ux[c] + rho * dxx) / rho;
+    const float ty = (w.Uy[c] + rho * dyx) / rho;
+    // sign(T) * clamp(|T| - 1/rho, min = 0)
+    const float zx = (tx > 0.f ? 1.f : (tx < 0.f ? -1.f : 0.f)) * fmaxf(fabsf(tx) - inv_rho, 0.f);
+    const float zy = (ty > 0.f ? 1.f : (ty < 0.f ? -1.f : 0.f)) * fmaxf(fabsf(ty) - inv_rho, 0.f);
+    w.Zx[c] = zx;
+    w.Zy[c] = zy;
+    w.Ux[c] = w.Ux[c] + rho * (dxx - zx);
+    w.Uy[c] = w.Uy[c] + rho * (dyx - zy);
+}
+
+// X sweep of iteration k: interior of X[(k+1)&1] from X[k&1], and per-block
+// sums of (X_new - X_old)^2.
+__global__ void __launch_bounds__(kTVBx * kTVBy) k_tv_x(TVWork w, int k, float lam, float lr,
+                                                       float den, float rho) {
+    if (w.flags[0]) return;
+    const int j = blockIdx.x * kTVBx + threadIdx.x, i = blockIdx.y * kTVBy + threadIdx.y;
+    double d2 = 0.0;
+    if (i >= 1 && i < w.M2 - 1 && j >= 1 && j < w.N2 - 1)
+        d2 = tv_x_cell(w, w.X[k & 1], w.X[(k + 1) & 1], i, j, lam, lr, den, rho);
+    // block sum (fixed order); the next Z/U sweep adds the block sums in order
+    __shared__ double red[kTVBx * kTVBy];
+    const int t = threadIdx.y * kTVBx + threadIdx.x;
+    red[t] = d2;
+    __syncthreads();
+    for (int s = kTVBx * kTVBy / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) w.part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Stopping test of iteration k: ||X_k - X_{k-1}|| < tol from the X sweep's
+// block sums, added in fixed order by one workgroup (the first workgroup of
+// the Z/U sweep that follows; a last-block pattern inside the X sweep would
+// need device-scope fences, which write back L2 across the XCDs).
+__device__ void tv_check(const TVWork& w, int k, int nparts, double tol) {
+    __shared__ double red[kTVBx * kTVBy];
+    const int t = threadIdx.y * kTVBx + threadIdx.x;
+    double acc = 0.0;
+    for (int b = t; b < nparts; b += kTVBx * kTVBy) acc += w.part[b];
+    red[t] = acc;
+    __syncthreads();
+    for (int s = kTVBx * kTVBy / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0 && sqrt(red[0]) < tol) { w.flags[1] = k; w.flags[0] = 1; }
+}
+
+// Split-variable and multiplier update of iteration k from X[(k+1)&1]
+// (periodic backward differences over the padded grid).
+__global__ void __launch_bounds__(kTVBx * kTVBy) k_tv_zu(TVWork w, int k, float rho, float inv_rho,
+                                                        int nparts, double tol) {
+    if (w.flags[0]) return;
+    if (blockIdx.x == 0 && blockIdx.y == 0) tv_check(w, k, nparts, tol);
+    const int j = blockIdx.x * kTVBx + threadIdx.x, i = blockIdx.y * kTVBy + threadIdx.y;
+    if (i >= w.M2 || j >= w.N2) return;
+    tv_zu_cell(w, w.X[(k + 1) & 1], i, j, rho, inv_rho);
+}
+
+__global__ void __launch_bounds__(kTVBx * kTVBy) k_tv_out(TVWork w, int num_iter, float* __restrict__ out) {
+    const int j = blockIdx.x * kTVBx + threadIdx.x, i = blockIdx.y * kTVBy + threadIdx.y;
+    const int M = w.M2 - 2, N = w.N2 - 2;
+    if (i >= M || j >= N) return;
+    const int kl = w.flags[0] ? w.flags[1] : num_iter - 1;    // last iteration run
+    const float* X = w.X[(kl + 1) & 1];
+    out[(size_t)i * N + j] = X[(size_t)(i + 1) * w.N2 + (j + 1)] * w.scal[0];
+}
+
+// ---------------------------------------------------------------------------
+// FIR low-pass of traces (multiscale.py:19-50): the reference reflect-pads
+// each trace by L-1 samples on the left and correlates with the taps, i.e.
+// out[t] = sum_k h[k] * x[|t + k - (L-1)|].  Traces are (S, nt, nrx) with the
+// receiver fastest, so a thread per (s, t, r) reads coalesced rows; taps are
+// summed in ascending order in FP32.
+__global__ void __launch_bounds__(256) k_fir(const float* __restrict__ x, int S, int nt, int nrx,
+                                             const float* __restrict__ h, int L,
+                                             float* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)S * nt * nrx) return;
+    const int r = (int)(e % nrx);
+    const int64_t st = e / nrx;
+    const int t = (int)(st % nt);
+    const int64_t base = (st - t) * nrx + r;          // (s, 0, r)
+    float acc = 0.f;
+    for (int k = 0; k < L; ++k) {
+        int i = t + k - (L - 1);
+        i = i < 0 ? -i : i;
+        acc = acc + h[k] * x[base + (int64_t)i * nrx];
+    }
+    out[e] = acc;
+}
+
+int64_t tv_blocks(int M, int N) {
+    return (int64_t)((N + 2 + kTVBx - 1) / kTVBx) * ((M + 2 + kTVBy - 1) / kTVBy);
+}
+
+int64_t tv_bytes(int M, int N) {
+    const int64_t plane = (int64_t)(M + 2) * (N + 2) * 4;
+    const int64_t pl = (plane + 255) / 256 * 256;
+    return 7 * pl + 256 + tv_blocks(M, N) * 8;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1584,6 +1767,59 @@ int fgpr_reduce_shots(const fgpr_problem* p, void* stream) {
     const int64_t ncell = (int64_t)p->NI * p->NJ;
     hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int64_t fgpr_tv_workspace_bytes(int M, int N) {
+    if (M < 1 || N < 1) return -1;
+    return tv_bytes(M, N);
+}
+
+int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+            void* workspace, int64_t bytes, float* out, void* stream) {
+    if (!data || !out || !workspace) return fail(FGPR_EINVAL, "null pointer");
+    if (M < 1 || N < 1 || num_iter < 1 || !(rho > 0.f)) return fail(FGPR_EINVAL, "bad TV arguments");
+    if (bytes < tv_bytes(M, N)) return fail(FGPR_EINVAL, "TV workspace too small");
+    if ((uintptr_t)workspace % 256) return fail(FGPR_EINVAL, "TV workspace not 256-byte aligned");
+    TVWork w;
+    w.M2 = M + 2; w.N2 = N + 2;
+    const int64_t plane = (int64_t)w.M2 * w.N2 * 4, pl = (plane + 255) / 256 * 256;
+    char* base = (char*)workspace;
+    float** planes[7] = {&w.X[0], &w.X[1], &w.Y, &w.Zx, &w.Zy, &w.Ux, &w.Uy};
+    for (int k = 0; k < 7; ++k) *planes[k] = (float*)(base + k * pl);
+    w.scal = (float*)(base + 7 * pl);
+    w.flags = (int*)(base + 7 * pl + 64);
+    w.part = (double*)(base + 7 * pl + 256);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 blk(kTVBx, kTVBy), grid((w.N2 + kTVBx - 1) / kTVBx, (w.M2 + kTVBy - 1) / kTVBy);
+    // float32 constants as the reference forms them: lamda * rho and
+    // 1 + 4 * lamda * rho in double, then rounded; 1 / rho likewise
+    const float lr = (float)((double)lamda * (double)rho);
+    const float den = (float)(1.0 + 4.0 * (double)lamda * (double)rho);
+    const float inv_rho = (float)(1.0 / (double)rho);
+    hipLaunchKernelGGL(k_tv_max, dim3(1), dim3(1024), 0, st, data, (int64_t)M * N, w);
+    hipLaunchKernelGGL(k_tv_init, grid, blk, 0, st, data, w);
+    const int nparts = (int)tv_blocks(M, N);
+    for (int k = 0; k < num_iter; ++k) {
+        hipLaunchKernelGGL(k_tv_x, grid, blk, 0, st, w, k, lamda, lr, den, rho);
+        hipLaunchKernelGGL(k_tv_zu, grid, blk, 0, st, w, k, rho, inv_rho, nparts, (double)tol);
+    }
+    hipLaunchKernelGGL(k_tv_out, dim3((N + kTVBx - 1) / kTVBx, (M + kTVBy - 1) / kTVBy), blk, 0,
+                       st, w, num_iter, out);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, float* out,
+             void* stream) {
+    if (!x || !taps || !out) return fail(FGPR_EINVAL, "null pointer");
+    if (S < 1 || nt < 1 || nrx < 1 || L < 1) return fail(FGPR_EINVAL, "bad FIR shape");
+    if (L - 1 >= nt) return fail(FGPR_EINVAL, "FIR reflect padding (taps - 1) must be shorter than the trace");
+    if (x == out) return fail(FGPR_EINVAL, "FIR input and output must not alias");
+    const int64_t n = (int64_t)S * nt * nrx;
+    hipLaunchKernelGGL(k_fir, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       x, S, nt, nrx, taps, L, out);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
@@ -92,6 +92,12 @@ SIGNATURES = {
     "fgpr_tb_settle": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p]),
     "fgpr_reduce_shots": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
+    "fgpr_tv_workspace_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
+    "fgpr_fir": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "fgpr_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                               ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 

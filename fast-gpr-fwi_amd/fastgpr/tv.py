@@ -6,8 +6,14 @@ image, zero-padded by one cell, with periodic forward differences D and
 soft-thresholding of the split variables.  The fixed-point sweep, defaults
 (lamda = 0.02, rho = 1, 500 iterations, stop when ||X - X_prev|| < 1e-5)
 and the float32 working precision are those of the reference.
+
+Device tensors run the HIP implementation (``fgpr_tv``: every iteration and
+the stopping test on the device, no host synchronisation per iteration);
+``total_variation_torch`` is the same iteration in torch operations.
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
@@ -31,8 +37,28 @@ def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
     """Denoise a (M, N, 1) or (M, N) model; returns (M, N, 1).
 
     Like the reference, the trailing singleton axis of ``data`` is squeezed
-    in place.
+    in place.  Device tensors go through the HIP kernels.
     """
+    if not data.is_cuda:
+        return total_variation_torch(data, lamda, rho, num_iter, tol)
+    from . import _native
+    from .solver import _stream
+    data.squeeze_(-1)
+    x = data.detach().to(torch.float32).contiguous()
+    M, N = x.shape
+    nbytes = int(_native.lib.fgpr_tv_workspace_bytes(M, N))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    out = torch.empty((M, N), dtype=torch.float32, device=x.device)
+    _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
+                                      float(rho), int(num_iter), float(tol),
+                                      ctypes.c_void_p(work.data_ptr()), nbytes,
+                                      ctypes.c_void_p(out.data_ptr()), _stream(x.device)), "tv")
+    return out.unsqueeze(-1)
+
+
+def total_variation_torch(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
+                          num_iter: int = 500, tol: float = 1e-5) -> torch.Tensor:
+    """The same iteration in torch operations (any device)."""
     data.squeeze_(-1)
     dmax = data.max()
     y = data / (dmax + 1e-6)

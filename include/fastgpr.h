@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 6
+#define FGPR_ABI_VERSION 7
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -233,6 +233,26 @@ int fgpr_tb_launch_table(const void* dev_table, int nitems, int total_wgs, int t
                          int tb_k, int adj, void* stream);
 int fgpr_tb_settle(const fgpr_problem* p, int parity, void* stream);
 int fgpr_reduce_shots(const fgpr_problem* p, void* stream);
+
+/* Total-variation denoiser, replaces total_variation (src/tv.py:135-217) as
+ * called from GPRFWI.backward (compute.py:916-933): ADMM on data / (max + 1e-6)
+ * zero-padded by one cell, lamda, rho, at most num_iter iterations, stopping
+ * when ||X - X_prev||_2 < tol; out = X * max.  data and out are M*N floats
+ * (row-major, the model without its trailing unit axis).  The iterations run
+ * on the device without host synchronisation; workspace is caller-owned
+ * device memory of fgpr_tv_workspace_bytes(M, N) bytes, 256-byte aligned. */
+int64_t fgpr_tv_workspace_bytes(int M, int N);
+int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+            void* workspace, int64_t bytes, float* out, void* stream);
+
+/* FIR low-pass of traces, replaces the conv1d of apply_filter
+ * (multiscale.py:19-50): x and out are S*nt*nrx floats laid out [s][t][r]
+ * (the layout of `traces`), taps has L floats (design_fir_filter,
+ * multiscale.py:3-17); each trace is reflect-padded by L-1 samples on the left,
+ * so out[s][t][r] = sum_k taps[k] * x[s][|t + k - (L-1)|][r].  Requires L-1 < nt
+ * (the reference's padding raises otherwise) and out != x. */
+int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, float* out,
+             void* stream);
 
 /* Release every HIP graph the library captured (also done at unload). */
 int fgpr_release_graphs(void);
