@@ -19,8 +19,11 @@
 // for every shot, and the FP64 gradient accumulation over shots needs no
 // atomics.  The CPML correction, source injection, receiver sampling and
 // snapshot store are fused into the two half-step kernels.  Floating-point
-// contraction is off in this file so each expression rounds exactly as the
-// reference's FP32 operation order does.
+// contraction is off in this file so each expression rounds as the
+// reference's FP32 operation order does, with one deliberate exception: the
+// CPML derivatives (a - b) / dx are formed as (a - b) * fl32(1 / dx) (at most
+// one unit in the last place apart; an IEEE division costs about ten
+// instructions in the CPML cells, 13-15 % of a time-blocked launch).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -64,6 +67,7 @@ struct Dev {
     int NI, NJ, P, S, nt, nsrc, nrx, pitch;
     int snap_base, snap_mod;
     float dx;
+    float idx;       // fl32(1 / dx): CPML derivatives are (a - b) * idx
     const float* __restrict__ coef;
     const float* __restrict__ pml;
     const int* __restrict__ src;
@@ -85,6 +89,7 @@ Dev to_dev(const fgpr_problem& p) {
     Dev d;
     d.NI = p.NI; d.NJ = p.NJ; d.P = p.P; d.S = p.S; d.nt = p.nt;
     d.nsrc = p.nsrc; d.nrx = p.nrx; d.pitch = p.pitch; d.dx = p.dx;
+    d.idx = 1.0f / p.dx;
     d.snap_base = p.snap_base;
     d.snap_mod = p.snap_mod;
     d.coef = p.coef; d.pml = p.pml; d.src = p.src; d.srcscale = p.srcscale; d.rx = p.rx;
@@ -211,7 +216,6 @@ k_h_phase(Dev d, int koff, int dir) {
         yRA01 = d.pml[4 * P + yq] - 1.0f; yRB = d.pml[5 * P + yq];
         yRE = d.pml[6 * P + yq]; yRF = d.pml[7 * P + yq];
     }
-    const float dx = d.dx;
 
     for (int s = 0; s < d.S; ++s) {
         const size_t b = (size_t)s * pl + o;
@@ -225,7 +229,7 @@ k_h_phase(Dev d, int koff, int dir) {
             if (yq >= 0) {
                 float* pp = psi_y(d, yk, s, i, yq);
                 const float ps = *pp;
-                const float dd = (ezj - ez) / dx;
+                const float dd = (ezj - ez) * d.idx;
                 hx = hx - uH4 * (yRA01 * dd + yRB * ps);
                 *pp = yRE * ps - yRF * dd;
             }
@@ -238,7 +242,7 @@ k_h_phase(Dev d, int koff, int dir) {
             if (xq >= 0) {
                 float* pp = psi_x(d, xk, s, xq, j);
                 const float ps = *pp;
-                const float dd = (ezi - ez) / dx;
+                const float dd = (ezi - ez) * d.idx;
                 hy = hy + uH4 * (xRA01 * dd + xRB * ps);
                 *pp = xRE * ps - xRF * dd;
             }
@@ -283,7 +287,6 @@ k_e_phase(Dev d, int koff, int dir) {
         yRA01 = d.pml[yq] - 1.0f; yRB = d.pml[P + yq];
         yRE = d.pml[2 * P + yq]; yRF = d.pml[3 * P + yq];
     }
-    const float dx = d.dx;
     const bool main_upd = (i >= 1 && j >= 1);
     double acc_e = 0.0, acc_s = 0.0;
 
@@ -297,8 +300,8 @@ k_e_phase(Dev d, int koff, int dir) {
         if (main_upd) ez = (uE0 * ez + uE1 * (hy - hyim)) - uE1 * (hx - hxjm);
         // x-CPML corrects Ez through dHy/dx, y-CPML through dHx/dy.
         float xcorr_d = 0.f, ycorr_d = 0.f;
-        if (xq >= 0) xcorr_d = (hy - hyim) / dx;
-        if (yq >= 0) ycorr_d = (hx - hxjm) / dx;
+        if (xq >= 0) xcorr_d = (hy - hyim) * d.idx;
+        if (yq >= 0) ycorr_d = (hx - hxjm) * d.idx;
         if (xk == EX0 && xq >= 0) {
             float* pp = psi_x(d, EX0, s, xq, j);
             const float ps = *pp;
@@ -451,7 +454,7 @@ k_xadj_b(Dev d, float* scratch) {
     const size_t o = (size_t)i * pitch + j;
     const float uH1 = d.coef[3 * pl + o];
     const float uH4 = d.coef[4 * pl + o];
-    const float idx = 1.0f / d.dx;
+    const float idx = d.idx;
     int xq = -1, xk = 0, yq = -1, yk = 0;
     if (P > 0) {
         if (i < P) { xq = P - 1 - i; xk = HX0; }
@@ -502,7 +505,7 @@ k_xadj_c(Dev d, float* scratch) {
     const size_t pl = plane(d);
     const size_t o = (size_t)i * pitch + j;
     const float c0 = (i >= 1 && j >= 1) ? d.coef[o] : 1.0f;
-    const float idx = 1.0f / d.dx;
+    const float idx = d.idx;
     for (int s = 0; s < d.S; ++s) {
         const size_t b = (size_t)s * pl + o;
         float v = c0 * d.Ez[b] + x.Vx[b] - x.Vy[b] - x.Mx[b] * idx - x.My[b] * idx;
@@ -773,7 +776,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         }
     }
 
-    const float dx = d.dx;
+    const float idx = d.idx;
     double ae[CPT], as_[CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; }
@@ -795,7 +798,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 if (gi >= 1) v = v - cH1[m] * (ezj - e);
                 if (yqh >= 0) {
                     const float c4 = cH4[m];
-                    const float dd = (ezj - e) / dx;
+                    const float dd = (ezj - e) * idx;
                     const float ph = sPsi[2][o];
                     v = v - c4 * (yh[0] * dd + yh[1] * ph);
                     sPsi[2][o] = yh[2] * ph - yh[3] * dd;
@@ -808,7 +811,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 if (gj >= 1) v = v + cH1[m] * (ezi - e);
                 if (q >= 0) {
                     const float c4 = cH4[m];
-                    const float dd = (ezi - e) / dx;
+                    const float dd = (ezi - e) * idx;
                     const float ph = sPsi[0][o];
                     v = v + c4 * (XH(m, q, 0) * dd + XH(m, q, 1) * ph);
                     sPsi[0][o] = XH(m, q, 2) * ph - XH(m, q, 3) * dd;
@@ -828,8 +831,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             const float c4 = cE4[m];
             const bool xminus = gi <= P;
             float dxd = 0.f, dyd = 0.f;
-            if (q >= 0) dxd = (hy[m] - hyim) / dx;
-            if (yqe >= 0) dyd = (hx[m] - hxjm) / dx;
+            if (q >= 0) dxd = (hy[m] - hyim) * idx;
+            if (yqe >= 0) dyd = (hx[m] - hxjm) * idx;
             if (q >= 0 && xminus) {
                 const float p = sPsi[1][o];
                 v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
@@ -1127,6 +1130,7 @@ TBKernel tb_kernel(int ri, int nt, int k, bool adj) {
     FGPR_TB_CASE(64, 256, 2) FGPR_TB_CASE(64, 256, 4) FGPR_TB_CASE(64, 256, 8)
     FGPR_TB_CASE(32, 512, 2) FGPR_TB_CASE(32, 512, 4) FGPR_TB_CASE(32, 512, 6)
     FGPR_TB_CASE(64, 1024, 2) FGPR_TB_CASE(64, 1024, 4) FGPR_TB_CASE(64, 1024, 6)
+    FGPR_TB_CASE(64, 1024, 8)
     return nullptr;
 }
 #undef FGPR_TB_CASE

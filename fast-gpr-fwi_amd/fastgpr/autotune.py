@@ -20,7 +20,7 @@ CANDIDATES = ([(32, 256, k) for k in (1, 2, 3, 4, 6)]
               + [(64, 512, k) for k in (1, 2, 3, 4, 6, 8)]
               + [(64, 256, k) for k in (2, 4, 8)]
               + [(32, 512, k) for k in (2, 4, 6)]
-              + [(64, 1024, k) for k in (2, 4, 6)])
+              + [(64, 1024, k) for k in (2, 4, 6, 8)])
 
 _cache: dict = {}
 
