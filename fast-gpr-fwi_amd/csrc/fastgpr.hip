@@ -782,7 +782,25 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; }
     __syncthreads();
 
-    // H half-step of the owned cells.
+    // Row-uniform facts of each owned row: gi >= 1, x-slab indices for H and
+    // E (-1 outside the slabs and outside the domain), minus-side flag.
+    bool rge1[CPT], xminus[CPT];
+    int qh_m[CPT], qe_m[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int gi = gi0 + r0 + m;
+        const bool in = P > 0 && gi >= 0 && gi < NI;
+        rge1[m] = gi >= 1;
+        qh_m[m] = in ? xq_h(gi) : -1;
+        qe_m[m] = in ? xq_e(gi) : -1;
+        xminus[m] = gi <= P;
+    }
+
+    // General H half-step.  Cells outside the domain (and the padding row and
+    // column) have zero coefficients and zero fields, so the plain update
+    // leaves them at zero; only the reference's row-0 / column-0 exceptions
+    // and the CPML terms need conditions (rows: uniform per wave; columns:
+    // per lane).
     auto h_half = [&]() {
         const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
 #pragma unroll
@@ -790,68 +808,61 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             const float e = ez[m];
             const float ezj = lane_next(e);
             const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below;
-            if (!(act & (1u << m))) continue;
-            const int r = r0 + m, gi = gi0 + r;
-            const int o = r * 64 + lane;
-            if (gi >= 1 || yqh >= 0) {
-                float v = hx[m];
-                if (gi >= 1) v = v - cH1[m] * (ezj - e);
-                if (yqh >= 0) {
-                    const float c4 = cH4[m];
-                    const float dd = (ezj - e) * idx;
-                    const float ph = sPsi[2][o];
-                    v = v - c4 * (yh[0] * dd + yh[1] * ph);
-                    sPsi[2][o] = yh[2] * ph - yh[3] * dd;
-                }
-                hx[m] = v;
+            const int o = (r0 + m) * 64 + lane;
+            float vx = hx[m];
+            if (rge1[m]) vx = vx - cH1[m] * (ezj - e);
+            float vy = hy[m] + cH1[m] * (ezi - e);
+            vy = (gj >= 1) ? vy : hy[m];
+            if (yqh >= 0) {
+                const float dd = (ezj - e) * idx;
+                const float ph = sPsi[2][o];
+                vx = vx - cH4[m] * (yh[0] * dd + yh[1] * ph);
+                sPsi[2][o] = yh[2] * ph - yh[3] * dd;
             }
-            const int q = P > 0 ? xq_h(gi) : -1;
-            if (gj >= 1 || q >= 0) {
-                float v = hy[m];
-                if (gj >= 1) v = v + cH1[m] * (ezi - e);
-                if (q >= 0) {
-                    const float c4 = cH4[m];
-                    const float dd = (ezi - e) * idx;
-                    const float ph = sPsi[0][o];
-                    v = v + c4 * (XH(m, q, 0) * dd + XH(m, q, 1) * ph);
-                    sPsi[0][o] = XH(m, q, 2) * ph - XH(m, q, 3) * dd;
-                }
-                hy[m] = v;
+            const int q = qh_m[m];
+            if (q >= 0) {
+                const float dd = (ezi - e) * idx;
+                const float ph = sPsi[0][o];
+                vy = vy + cH4[m] * (XH(m, q, 0) * dd + XH(m, q, 1) * ph);
+                sPsi[0][o] = XH(m, q, 2) * ph - XH(m, q, 3) * dd;
             }
+            hx[m] = vx;
+            hy[m] = vy;
         }
         sHyBot[w * 64 + lane] = hy[CPT - 1];
     };
 
-    // E update of owned cell m (main + CPML in slab order x0, y0, xmax, ymax).
-    auto e_cell = [&](int m, int gi, int o, float hyim, float hxjm) -> float {
-        float v = ez[m];
-        if (gi >= 1 && gj >= 1) v = (cE0[m] * v + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
-        const int q = P > 0 ? xq_e(gi) : -1;
-        if (q >= 0 || yqe >= 0) {
-            const float c4 = cE4[m];
-            const bool xminus = gi <= P;
-            float dxd = 0.f, dyd = 0.f;
-            if (q >= 0) dxd = (hy[m] - hyim) * idx;
-            if (yqe >= 0) dyd = (hx[m] - hxjm) * idx;
-            if (q >= 0 && xminus) {
+    // General E update of owned cell m: plain update, the row-0 / column-0
+    // exceptions, then the CPML terms in slab order x0, y0, xmax, ymax.
+    auto e_cell = [&](int m, int o, float hyim, float hxjm) -> float {
+        const float e0 = ez[m];
+        float v = (cE0[m] * e0 + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+        if (!rge1[m]) v = e0;
+        v = (gj >= 1) ? v : e0;
+        const float c4 = cE4[m];
+        auto yterm = [&]() {
+            const float dyd = (hx[m] - hxjm) * idx;
+            const float p = sPsi[3][o];
+            v = v - c4 * (ye[0] * dyd + ye[1] * p);
+            sPsi[3][o] = ye[2] * p - ye[3] * dyd;
+        };
+        const int q = qe_m[m];
+        if (q < 0) {
+            if (yqe >= 0) yterm();
+        } else {
+            const float dxd = (hy[m] - hyim) * idx;
+            auto xterm = [&]() {
                 const float p = sPsi[1][o];
                 v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
                 sPsi[1][o] = XE(m, q, 2) * p - XE(m, q, 3) * dxd;
-            }
-            if (yqe >= 0 && ye_lo) {
-                const float p = sPsi[3][o];
-                v = v - c4 * (ye[0] * dyd + ye[1] * p);
-                sPsi[3][o] = ye[2] * p - ye[3] * dyd;
-            }
-            if (q >= 0 && !xminus) {
-                const float p = sPsi[1][o];
-                v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
-                sPsi[1][o] = XE(m, q, 2) * p - XE(m, q, 3) * dxd;
-            }
-            if (yqe >= 0 && !ye_lo) {
-                const float p = sPsi[3][o];
-                v = v - c4 * (ye[0] * dyd + ye[1] * p);
-                sPsi[3][o] = ye[2] * p - ye[3] * dyd;
+            };
+            if (xminus[m]) {
+                xterm();
+                if (yqe >= 0) yterm();
+            } else {
+                if (yqe >= 0 && ye_lo) yterm();
+                xterm();
+                if (yqe >= 0 && !ye_lo) yterm();
             }
         }
         return v;
@@ -924,9 +935,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
                         v = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
                     } else {
-                        if (!(act & (1u << m))) continue;
-                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1) ? hy_above : 0.f);
-                        v = e_cell(m, gi, r * 64 + lane, hyim, (gj >= 1) ? hxjm : 0.f);
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
+                        v = e_cell(m, r * 64 + lane, hyim, hxjm);
                     }
                     if (srcbits & (1u << m)) {
                         for (int kk = 0; kk < d.nsrc; ++kk) {
@@ -935,7 +945,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         }
                     }
                     ez[m] = v;
-                    if (snap_thr && r >= K && r < K + TI && centre_col) *snap_at(n, m) = v;
+                    if (snap_thr && (FAST || (act & (1u << m))) && r >= K && r < K + TI && centre_col)
+                        *snap_at(n, m) = v;
                 }
                 sEzTop[w * 64 + lane] = ez[0];
                 __syncthreads();
@@ -961,15 +972,14 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 #pragma unroll
                 for (int m = 0; m < CPT; ++m) {
                     const float hxjm = lane_prev(hx[m]);
-                    const int r = r0 + m, gi = gi0 + r;
+                    const int r = r0 + m;
                     float lam;
                     if (FAST) {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
                         lam = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
                     } else {
-                        if (!(act & (1u << m))) continue;
-                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : ((gi >= 1) ? hy_above : 0.f);
-                        lam = e_cell(m, gi, r * 64 + lane, hyim, (gj >= 1) ? hxjm : 0.f);
+                        const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
+                        lam = e_cell(m, r * 64 + lane, hyim, hxjm);
                     }
                     ez[m] = lam;
                     if (r >= K && r < K + TI && centre_col) {
