@@ -1163,6 +1163,16 @@ TBMultiKernel tb_multi_kernel(int ri, int nt, int k, bool adj) {
 
 bool tb_enabled(const fgpr_problem& p) { return p.tb_k > 0; }
 
+// Steps per captured graph on the time-blocked path: whole pairs of launches
+// (the ping-pong parity returns to 0), and enough launches that graph
+// replay and the counter update stay small next to the kernels (16 launches).
+int tb_chunk(int chunk, int k) {
+    if (chunk <= 0) return chunk;
+    const int pair = 2 * k;
+    chunk = chunk < 16 * k ? 16 * k : chunk;
+    return (chunk + pair - 1) / pair * pair;
+}
+
 int tb_check(const fgpr_problem& p, bool adj) {
     if (!tb_kernel(p.tb_ri, p.tb_nt, p.tb_k, adj))
         return fail(FGPR_EINVAL, "unsupported time-blocking variant (tb_ri, tb_nt, tb_k)");
@@ -1543,7 +1553,7 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
     hipStream_t st = (hipStream_t)stream;
     fgpr_problem q = *p;
     int n = n0;
-    if (tb && chunk % (2 * p->tb_k)) chunk -= chunk % (2 * p->tb_k);
+    if (tb) chunk = tb_chunk(chunk, p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n1 - n0 >= chunk) {
         hipGraphExec_t ex;
         rc = get_graph(q, 0, chunk, &ex);
@@ -1612,7 +1622,7 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
     hipStream_t st = (hipStream_t)stream;
     fgpr_problem q = *p;
     int n = n_hi;
-    if (tb && chunk % (2 * p->tb_k)) chunk -= chunk % (2 * p->tb_k);
+    if (tb) chunk = tb_chunk(chunk, p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n_hi - n_lo + 1 >= chunk) {
         hipGraphExec_t ex;
         rc = get_graph(q, 1, chunk, &ex);
