@@ -90,7 +90,8 @@ def test_c1_traces_and_gradient(hip_device):
         assert e <= GRAD_TOL_FP64
 
 
-TB_VARIANTS = [(32, 256, 1), (32, 256, 4), (64, 512, 3), (64, 512, 8), (64, 256, 4)]
+TB_VARIANTS = [(32, 256, 1), (32, 256, 4), (64, 512, 3), (64, 512, 8), (64, 256, 4), (32, 512, 4),
+               (64, 1024, 6), (64, 1024, 8)]
 
 
 @pytest.mark.parametrize("tb", TB_VARIANTS)
