@@ -492,11 +492,16 @@ def serve_bench(dev, seconds, rate):
     # alone vs in-mix on a sample of the completed requests
     worst = 0.0
     done = [r for r in reqs if r.result and "er" in r.result]
+    budget = srv.budget
+    del srv
+    torch.cuda.empty_cache()
     for r in done[:: max(1, len(done) // 8)][:8]:
-        a = Server(dev)
+        a = Server(dev, hbm_budget=budget)
         ra = a.submit(Request(r.rid, r.wl, r.model, r.obs, tb=r.tb))
         a.run()
         torch.cuda.synchronize()
+        if "er" not in ra.result:
+            raise RuntimeError(f"request {r.rid} served alone failed: {ra.result}")
         for k in ("er", "se", "traces"):
             x, y = r.result[k].double(), ra.result[k].double()
             worst = max(worst, float((x - y).abs().max() / max(y.abs().max(), 1e-30)))
