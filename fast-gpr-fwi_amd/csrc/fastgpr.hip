@@ -66,6 +66,7 @@ enum Slab { EX0 = 0, EXM = 1, HX0 = 2, HXM = 3, EY0 = 4, EYM = 5, HY0 = 6, HYM =
 struct Dev {
     int NI, NJ, P, S, nt, nsrc, nrx, pitch;
     int snap_base, snap_mod;
+    int snap_tiled;  // snapshots in the time-blocked layout (tb_snap_off)
     float dx;
     float idx;       // fl32(1 / dx): CPML derivatives are (a - b) * idx
     const float* __restrict__ coef;
@@ -92,6 +93,7 @@ Dev to_dev(const fgpr_problem& p) {
     d.idx = 1.0f / p.dx;
     d.snap_base = p.snap_base;
     d.snap_mod = p.snap_mod;
+    d.snap_tiled = p.snap_tiled;
     d.coef = p.coef; d.pml = p.pml; d.src = p.src; d.srcscale = p.srcscale; d.rx = p.rx;
     d.Ez = p.Ez; d.Hx = p.Hx; d.Hy = p.Hy; d.psi = p.psi;
     d.traces = p.traces; d.snaps = p.snaps; d.resid = p.resid;
@@ -775,6 +777,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             if (sc[1] == gj && li >= 0 && li < CPT) srcbits |= 1u << li;
         }
     }
+    const bool src_any = __ballot(srcbits != 0u) != 0ull;   // wave-uniform
 
     const float idx = d.idx;
     double ae[CPT], as_[CPT];
@@ -885,26 +888,78 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         sHyBot[w * 64 + lane] = hy[CPT - 1];
     };
 
-    // Snapshot of (step n, this thread's row m): snap_thr + slot(n) * snap_slot + m * pitch.
+    // Snapshot of (step n, this thread's row m) = snap_wave[slot(n) * snap_slot + m * pitch + gj]:
+    // a wave-uniform base plus a per-lane offset; slots advance incrementally
+    // (a ring of snap_mod slots, or n - snap_base), never by a division per step.
     const int64_t snap_slot = (int64_t)d.S * NI * pitch;
-    float* const snap_thr = d.snaps ? d.snaps + ((int64_t)s * NI + gi0 + r0) * pitch + gj : nullptr;
-    auto snap_at = [&](int n, int m) -> float* {
-        const int slot = d.snap_mod ? n % d.snap_mod : n - d.snap_base;
-        return snap_thr + slot * snap_slot + (int64_t)m * pitch;
+    float* const snap_wave = d.snaps ? d.snaps + ((int64_t)s * NI + gi0 + r0) * pitch : nullptr;
+    auto slot_of = [&](int n) -> int { return d.snap_mod ? n % d.snap_mod : n - d.snap_base; };
+    auto slot_next = [&](int sl) -> int { return (d.snap_mod && sl + 1 == d.snap_mod) ? 0 : sl + 1; };
+    auto slot_prev = [&](int sl) -> int { return (d.snap_mod && sl == 0) ? d.snap_mod - 1 : sl - 1; };
+    auto snap_ref = [&](int sl, int m) -> float& {
+        return (snap_wave + (int64_t)sl * snap_slot + m * pitch)[gj];
+    };
+    // Time-blocked layout: per (slot, shot, tile) a block of RI x 64 floats
+    // ordered [wave][lane][row], so each thread's CPT rows are contiguous
+    // (one vector store / load per step) and each block is written by one
+    // workgroup.  Only tile-centre cells are ever read back.
+    constexpr int kBlk = RI * 64;
+    const bool tiled = d.snap_tiled != 0;
+    const int64_t tslot = (int64_t)d.S * ntiles * kBlk;
+    float* const tsnap = d.snaps ? d.snaps + ((int64_t)s * ntiles + tile) * kBlk + (w * 64 + lane) * CPT
+                                 : nullptr;
+    const bool wave_centre = r0 + CPT > K && r0 < K + TI;          // wave-uniform
+    auto tstore = [&](int sl) {
+        float* q = tsnap + sl * tslot;
+#pragma unroll
+        for (int m = 0; m < CPT; m += 4) {
+            if constexpr (CPT % 4 == 0)
+                *(float4*)(q + m) = make_float4(ez[m], ez[m + 1], ez[m + 2], ez[m + 3]);
+        }
+        if constexpr (CPT % 4 != 0)
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) q[m] = ez[m];
+    };
+    auto tload = [&](int sl, float* f) {
+        const float* q = tsnap + sl * tslot;
+#pragma unroll
+        for (int m = 0; m < CPT; m += 4) {
+            if constexpr (CPT % 4 == 0) {
+                const float4 v = *(const float4*)(q + m);
+                f[m] = v.x; f[m + 1] = v.y; f[m + 2] = v.z; f[m + 3] = v.w;
+            }
+        }
+        if constexpr (CPT % 4 != 0)
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) f[m] = q[m];
     };
 
     // adjoint: snapshots of steps n (fcur) and n - 1 (fprv) of the current step
     float fcur[CPT], fprv[CPT];
 #pragma unroll
-    for (int m = 0; m < CPT; ++m) {
-        const int r = r0 + m;
-        const bool g = ADJ && nsteps > 0 && (act & (1u << m)) && r >= K && r < K + TI && centre_col;
-        fcur[m] = g ? *snap_at(n0, m) : 0.f;
-        fprv[m] = g ? *snap_at(n0 - 1, m) : 0.f;
+    for (int m = 0; m < CPT; ++m) { fcur[m] = 0.f; fprv[m] = 0.f; }
+    if (ADJ && nsteps > 0) {
+        if (tiled) {
+            if (wave_centre && centre_col) {
+                tload(slot_of(n0), fcur);
+                tload(slot_of(n0 - 1), fprv);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < CPT; ++m) {
+                const int r = r0 + m;
+                if ((act & (1u << m)) && r >= K && r < K + TI && centre_col) {
+                    fcur[m] = snap_ref(slot_of(n0), m);
+                    fprv[m] = snap_ref(slot_of(n0 - 1), m);
+                }
+            }
+        }
     }
 
     auto steps = [&](auto fast_tag) {
         constexpr bool FAST = decltype(fast_tag)::value;
+        // forward: slot of step n; adjoint: slot of step n - 2 (the prefetch)
+        int sl = ADJ ? (nsteps > 1 ? slot_of(n0 - 2) : 0) : (snap_wave ? slot_of(n0) : 0);
         for (int k = 0; k < nsteps; ++k) {
             const int n = ADJ ? n0 - k : n0 + k;
             if (!ADJ) {
@@ -938,17 +993,20 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
                         v = e_cell(m, r * 64 + lane, hyim, hxjm);
                     }
-                    if (srcbits & (1u << m)) {
+                    if (src_any && (srcbits & (1u << m))) {
                         for (int kk = 0; kk < d.nsrc; ++kk) {
                             const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
                             if (sc[0] == gi && sc[1] == gj) v = v - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
                         }
                     }
                     ez[m] = v;
-                    if (snap_thr && (FAST || (act & (1u << m))) && r >= K && r < K + TI && centre_col)
-                        *snap_at(n, m) = v;
+                    if (!tiled && snap_wave && (FAST || (act & (1u << m))) && r >= K && r < K + TI &&
+                        centre_col)
+                        snap_ref(sl, m) = v;
                 }
+                if (tiled && tsnap && wave_centre && centre_col) tstore(sl);
                 sEzTop[w * 64 + lane] = ez[0];
+                sl = slot_next(sl);
                 __syncthreads();
             } else {
                 // adjoint source (back.cu:24-37) at receivers inside the region,
@@ -991,12 +1049,20 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 }
                 // snapshots for the next step, loaded one step ahead
 #pragma unroll
-                for (int m = 0; m < CPT; ++m) {
-                    const int r = r0 + m;
-                    fcur[m] = fprv[m];
-                    if (k + 1 < nsteps && (FAST || (act & (1u << m))) && r >= K && r < K + TI && centre_col)
-                        fprv[m] = *snap_at(n - 2, m);
+                for (int m = 0; m < CPT; ++m) fcur[m] = fprv[m];
+                if (k + 1 < nsteps) {
+                    if (tiled) {
+                        if (wave_centre && centre_col) tload(sl, fprv);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < CPT; ++m) {
+                            const int r = r0 + m;
+                            if ((FAST || (act & (1u << m))) && r >= K && r < K + TI && centre_col)
+                                fprv[m] = snap_ref(sl, m);
+                        }
+                    }
                 }
+                sl = slot_prev(sl);
                 sEzTop[w * 64 + lane] = ez[0];
                 __syncthreads();
                 if (FAST) h_fast(); else h_half();
@@ -1069,6 +1135,27 @@ k_tb_multi(const unsigned char* __restrict__ table, int nitems) {
     const MultiEntry* ents = (const MultiEntry*)(table + tb_table_header(nitems));
     const MultiEntry& me = ents[lo];
     tb_body<RI, NT, K, ADJ>(me.a, me.n0, me.nsteps, b - prefix[lo], prefix[lo + 1] - prefix[lo]);
+}
+
+// Row-major Ez (S x (NI+1) x pitch) -> one slot of the time-blocked
+// snapshot layout of variant (RI, NT, K): seeds a checkpoint segment's ring
+// slot from a saved state.  Cells outside the domain are written as zero.
+__global__ void __launch_bounds__(256) k_snap_pack(Dev d, int RI, int NT, int K, int ti_count,
+                                                   int tj_count, int slot, const float* __restrict__ Ez) {
+    const int ntiles = ti_count * tj_count, blk = RI * 64, cpt = RI / (NT / 64);
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)d.S * ntiles * blk) return;
+    const int64_t b = e / blk;
+    const int rem = (int)(e - b * blk);
+    const int wl = rem / cpt, m = rem - wl * cpt;
+    const int w = wl >> 6, lane = wl & 63;
+    const int sh = (int)(b / ntiles), tile = (int)(b - (int64_t)sh * ntiles);
+    const int ti = tile / tj_count, tj = tile - ti * tj_count;
+    const int gi = ti * (RI - 2 * K - 1) - K + w * cpt + m;
+    const int gj = tj * (kTBCols - 2 * K - 1) - K + lane;
+    const bool in = gi >= 0 && gi < d.NI && gj >= 0 && gj < d.NJ;
+    d.snaps[(int64_t)slot * d.S * ntiles * blk + e] =
+        in ? Ez[((int64_t)sh * (d.NI + 1) + gi) * d.pitch + gj] : 0.f;
 }
 
 // grad[c] += sum_s gshot[p][s][c] in shot order (deterministic).
@@ -1310,6 +1397,8 @@ int check_problem(const fgpr_problem* p) {
     if (!p->coef || !p->Ez || !p->Hx || !p->Hy) return fail(FGPR_EINVAL, "null coef/field pointer");
     if (p->P > 0 && (!p->pml || !p->psi)) return fail(FGPR_EINVAL, "null pml/psi pointer");
     if (!(p->dx > 0.f)) return fail(FGPR_EINVAL, "dx must be positive");
+    if (p->snap_tiled && p->tb_k <= 0)
+        return fail(FGPR_EINVAL, "time-blocked snapshot layout (snap_tiled) needs the time-blocked path");
     return FGPR_OK;
 }
 
@@ -1585,6 +1674,8 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
                  void* stream) {
     int rc = check_problem(p);
     if (rc) return rc;
+    if (grad_mode == FGPR_GRAD_EXACT && p->snap_tiled)
+        return fail(FGPR_EINVAL, "the exact adjoint reads row-major snapshots (snap_tiled must be 0)");
     if (grad_mode != FGPR_GRAD_REFERENCE && grad_mode != FGPR_GRAD_EXACT)
         return fail(FGPR_EINVAL, "unknown gradient mode");
     const bool exact = grad_mode == FGPR_GRAD_EXACT;
@@ -1831,6 +1922,25 @@ int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_ite
     }
     hipLaunchKernelGGL(k_tv_out, dim3((N + kTVBx - 1) / kTVBx, (M + kTVBy - 1) / kTVBy), blk, 0,
                        st, w, num_iter, out);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int64_t fgpr_snap_slot_floats(const fgpr_problem* p) {
+    if (!p) return -1;
+    if (!p->snap_tiled) return (int64_t)p->S * p->NI * p->pitch;
+    return (int64_t)p->S * p->tb_ti * p->tb_tj * p->tb_ri * 64;
+}
+
+int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (!p->snap_tiled || !p->snaps || !Ez) return fail(FGPR_EINVAL, "snap_pack needs a tiled snapshot buffer and Ez");
+    if ((rc = tb_check(*p, false))) return rc;
+    if (slot < 0 || slot >= p->nsnap) return fail(FGPR_EINVAL, "snapshot slot outside the buffer");
+    const int64_t n = (int64_t)p->S * p->tb_ti * p->tb_tj * p->tb_ri * 64;
+    hipLaunchKernelGGL(k_snap_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       to_dev(*p), p->tb_ri, p->tb_nt, p->tb_k, p->tb_ti, p->tb_tj, slot, Ez);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
@@ -51,6 +51,7 @@ class Problem(ctypes.Structure):
         ("gshot", ctypes.c_void_p),
         ("scratch", ctypes.c_void_p),
         ("snap_base", ctypes.c_int32), ("nsnap", ctypes.c_int32), ("snap_mod", ctypes.c_int32),
+        ("snap_tiled", ctypes.c_int32),
     ]
 
 
@@ -93,6 +94,9 @@ SIGNATURES = {
     "fgpr_reduce_shots": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
     "fgpr_tv_workspace_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
+    "fgpr_snap_slot_floats": (ctypes.c_int64, [ctypes.POINTER(Problem)]),
+    "fgpr_snap_pack": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
     "fgpr_fir": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "fgpr_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,

@@ -32,11 +32,14 @@ MULTI_VARIANTS = [(32, 512, 2), (32, 512, 4), (32, 512, 6), (64, 1024, 2), (64, 
                   (64, 1024, 6), (64, 512, 2), (64, 512, 4), (64, 512, 6), (64, 512, 8)]
 
 
-def job_bytes(NI, NJ, S, nt, nrx, P, C, gradient=True) -> int:
+def job_bytes(NI, NJ, S, nt, nrx, P, C, gradient=True, tb=(64, 1024, 6)) -> int:
     """Device bytes one job holds (fields and ping-pong partners for the
-    forward and adjoint sides, CPML memory, coefficients, snapshot ring,
-    saved states, traces / residual, FP64 gradients)."""
+    forward and adjoint sides, CPML memory, coefficients, snapshot ring in the
+    time-blocked layout of variant ``tb``, saved states, traces / residual,
+    FP64 gradients)."""
     pitch = _native.pitch(NJ)
+    _, _, ti, tj = _native.tb_tiles(NI, NJ, *tb)
+    slot = S * max(ti * tj * tb[0] * 64, NI * pitch)
     field = 3 * S * (NI + 1) * pitch * 4
     psi = max(_native.psi_floats(NI, NJ, P, S), 1) * 4
     sides = 2 if gradient else 1
@@ -44,7 +47,7 @@ def job_bytes(NI, NJ, S, nt, nrx, P, C, gradient=True) -> int:
     b += 2 * S * nt * nrx * 4
     if gradient:
         nseg = (nt + C - 1) // C
-        b += (C + 1) * S * NI * pitch * 4
+        b += (C + 1) * slot * 4
         b += max(nseg - 1, 0) * (field + psi)
         b += 2 * S * NI * NJ * 8 + 2 * NI * NJ * 8 + 2 * NI * NJ * 4
     return int(b)
@@ -85,7 +88,7 @@ class Job:
             self._enter_phase()
             return
         self.obs = obs.detach().to(device=dev, dtype=torch.float32).contiguous()
-        self.ring = torch.empty((C + 1, S, NI, pitch), dtype=torch.float32, device=dev)
+        self.ring = torch.empty((C + 1,) + self.fb.snap_slot_shape(), dtype=torch.float32, device=dev)
         self.pf = self.fb.problem(self.coef_f, srcscale=self.scale)
         self.pf.snaps, self.pf.nsnap, self.pf.snap_mod = self.ring.data_ptr(), C + 1, C + 1
         self.rec_traces = torch.empty_like(self.fb.traces)
@@ -128,7 +131,8 @@ class Job:
                 k = a // self.C
                 self.states.restore(k, self.fb)
                 if k >= 1:
-                    self.ring[(a - 1) % (self.C + 1)].copy_(self.states.fields[k - 1][0][:, :self.fb.NI, :])
+                    self.fb.seed_slot(self.pf_rec, (a - 1) % (self.C + 1), self.ring,
+                                      self.states.fields[k - 1][0])
         else:
             self.n = a
             self.ab.gshot.zero_()

@@ -38,7 +38,11 @@ def interval(G, checkpoint) -> int | None:
         return None
     nt = G.iterations
     if checkpoint == "auto":
-        full = nt * G.S * G.NI * _native.pitch(G.NJ) * 4
+        slot = G.S * G.NI * _native.pitch(G.NJ)
+        if G.time_block and G.options.get("gradient", "reference") != "exact":
+            _, _, ti, tj = _native.tb_tiles(G.NI, G.NJ, *G.time_block)
+            slot = G.S * ti * tj * G.time_block[0] * 64          # time-blocked layout
+        full = nt * slot * 4
         free, _ = torch.cuda.mem_get_info(G.device)
         if full <= 0.5 * free:
             return None
@@ -112,7 +116,7 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
     C, nt, S, NI, NJ = st.C, b.nt, b.S, b.NI, b.NJ
     stream = _s._stream(dev)
     mode = _s._MODES[G.exec_mode]
-    ring = torch.empty((C + 1, S, NI, b.pitch), dtype=torch.float32, device=dev)
+    ring = torch.empty((C + 1,) + b.snap_slot_shape(), dtype=torch.float32, device=dev)
     traces_scratch = torch.empty_like(b.traces)
     pf = b.problem(fw.coef, srcscale=fw.scale)
     pf.snaps, pf.nsnap, pf.snap_mod = ring.data_ptr(), C + 1, C + 1
@@ -144,7 +148,7 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
         st.restore(k, b)
         if k >= 1:
             # Ez after step s0 - 1 is the saved state's Ez
-            ring[(s0 - 1) % (C + 1)].copy_(st.fields[k - 1][0][:, :NI, :])
+            b.seed_slot(pf, (s0 - 1) % (C + 1), ring, st.fields[k - 1][0])
         check(lib.fgpr_forward(ctypes.byref(pf), s0, e, mode, G.graph_chunk, stream), "recompute")
         n_lo = s0 if exact else max(s0, 1)
         if e - 1 >= n_lo:

@@ -103,13 +103,13 @@ class Server:
         tb = req.tb or self.variant_for(wl)
         K = tb[2]
         full = batching.job_bytes(wl.NI, wl.NJ, wl.S, wl.nt, wl.rx.shape[1], wl.P, wl.nt,
-                                  req.obs is not None)
+                                  req.obs is not None, tb)
         C = wl.nt
         if req.obs is not None and full > self.budget // 4:
             c = int(math.ceil(math.sqrt(3.0 * wl.nt)))
             C = max(2 * K, ((c + 2 * K - 1) // (2 * K)) * 2 * K)
         nbytes = batching.job_bytes(wl.NI, wl.NJ, wl.S, wl.nt, wl.rx.shape[1], wl.P, C,
-                                    req.obs is not None)
+                                    req.obs is not None, tb)
         return tb, (None if C >= wl.nt else C), nbytes
 
     def _admit(self):

@@ -103,7 +103,7 @@ class Batch:
     ``tb`` = (region rows, threads, steps per launch) selects the
     time-blocked kernels; None selects the per-step kernels."""
 
-    def __init__(self, G, store: bool = True, tb=None):
+    def __init__(self, G, store: bool = True, tb=None, tiled=None):
         dev = G.device
         self.G = G
         self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, G.S, G.iterations
@@ -123,9 +123,12 @@ class Batch:
         self.nrx = self.rx.shape[1]
         self.ctr = torch.zeros(1, dtype=torch.int32, device=dev)
         self.traces = torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev)
-        self.snaps = (torch.empty((self.nt, self.S, self.NI, self.pitch), dtype=torch.float32, device=dev)
-                      if store else None)
         self.tb = tuple(tb) if tb else None
+        # Snapshots in the time-blocked layout when both sweeps run the same
+        # time-blocked variant (the exact adjoint reads the row-major one).
+        if tiled is None:
+            tiled = bool(self.tb) and G.options.get("gradient", "reference") != "exact"
+        self.tiled = bool(tiled) and bool(self.tb)
         self.gshot = None
         if self.tb:
             self.fields2 = torch.zeros_like(self.fields)
@@ -135,6 +138,23 @@ class Batch:
             as_dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
             self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
                               as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
+        self.snaps = (torch.empty((self.nt,) + self.snap_slot_shape(), dtype=torch.float32, device=dev)
+                      if store else None)
+
+    def snap_slot_shape(self) -> tuple:
+        """Shape of one snapshot slot: (S, NI, pitch) row-major, or
+        (S, tiles * rows * 64) in the time-blocked layout."""
+        if self.tiled:
+            return (self.S, self.tiles[0] * self.tiles[1] * self.tb[0] * 64)
+        return (self.S, self.NI, self.pitch)
+
+    def seed_slot(self, p, slot: int, ring: torch.Tensor, ez: torch.Tensor):
+        """Write snapshot slot ``slot`` of ``ring`` (the buffer ``p.snaps``
+        points to) from a row-major Ez state (S, NI+1, pitch)."""
+        if self.tiled:
+            check(lib.fgpr_snap_pack(ctypes.byref(p), slot, _ptr(ez), _stream(ez.device)), "snap_pack")
+        else:
+            ring[slot].copy_(ez[:, :self.NI, :])
 
     def reset_state(self):
         self.fields.zero_()
@@ -172,6 +192,7 @@ class Batch:
         p.snap_base = 0
         p.snap_mod = 0
         p.nsnap = self.snaps.shape[0] if self.snaps is not None else 0
+        p.snap_tiled = 1 if self.tiled else 0
         p.resid = resid.data_ptr() if resid is not None else None
         p.grad_eps = grad_eps.data_ptr() if grad_eps is not None else None
         p.grad_sig = grad_sig.data_ptr() if grad_sig is not None else None

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 7
+#define FGPR_ABI_VERSION 8
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -119,6 +119,11 @@ typedef struct fgpr_problem {
     int32_t nsnap;              /* snapshot slots in `snaps` (nt when it holds every step) */
     int32_t snap_mod;           /* > 0: step n lives in slot n % snap_mod (a ring of segment
                                    slots; the same captured graph then serves every segment) */
+    int32_t snap_tiled;         /* 1: snapshots in the time-blocked layout of this problem's
+                                   variant -- per (slot, shot, tile) a block of tb_ri x 64 floats
+                                   ordered [wave][lane][row] -- written by the time-blocked
+                                   forward and read by the time-blocked adjoint of the same
+                                   variant; 0: the row-major layout above */
 } fgpr_problem;
 
 /* ABI version (FGPR_ABI_VERSION). */
@@ -244,6 +249,15 @@ int fgpr_reduce_shots(const fgpr_problem* p, void* stream);
 int64_t fgpr_tv_workspace_bytes(int M, int N);
 int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
             void* workspace, int64_t bytes, float* out, void* stream);
+
+/* Floats per snapshot slot in the problem's layout (row-major S*NI*pitch, or
+ * S*tiles*tb_ri*64 when snap_tiled). */
+int64_t fgpr_snap_slot_floats(const fgpr_problem* p);
+
+/* Writes slot `slot` of a tiled snapshot buffer from a row-major Ez
+ * (S*(NI+1)*pitch floats, the layout of p->Ez): seeds the ring slot before a
+ * checkpointed segment is recomputed.  No counterpart in the reference. */
+int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* stream);
 
 /* FIR low-pass of traces, replaces the conv1d of apply_filter
  * (multiscale.py:19-50): x and out are S*nt*nrx floats laid out [s][t][r]
