@@ -18,12 +18,15 @@
 // batch, so the per-model coefficients are loaded once per cell and reused
 // for every shot, and the FP64 gradient accumulation over shots needs no
 // atomics.  The CPML correction, source injection, receiver sampling and
-// snapshot store are fused into the two half-step kernels.  Floating-point
-// contraction is off in this file so each expression rounds as the
-// reference's FP32 operation order does, with one deliberate exception: the
-// CPML derivatives (a - b) / dx are formed as (a - b) * fl32(1 / dx) (at most
-// one unit in the last place apart; an IEEE division costs about ten
-// instructions in the CPML cells, 13-15 % of a time-blocked launch).
+// snapshot store are fused into the two half-step kernels.  Implicit
+// floating-point contraction is off in this file: the cell updates follow the
+// reference's FP32 operation order, with every product that feeds an add or a
+// subtract fused explicitly (fmaf, the helpers upd_* / cpml_*), as a CUDA
+// build with default flags contracts them, and the CPML derivatives
+// (a - b) / dx formed as (a - b) * fl32(1 / dx) (at most one unit in the last
+// place apart; an IEEE division costs about ten instructions in the CPML
+// cells).  The per-step and the time-blocked kernels share these helpers and
+// agree bitwise.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -119,6 +122,24 @@ __device__ __forceinline__ float* psi_x(const Dev& d, int kind, int s, int q, in
 __device__ __forceinline__ float* psi_y(const Dev& d, int kind, int s, int i, int q) {
     return d.psi + 4 * xslab_floats(d) + (size_t)(kind - 4) * yslab_floats(d) +
            ((size_t)s * d.NI + i) * d.P + q;
+}
+
+// Cell arithmetic shared by the per-step and the time-blocked kernels (one
+// definition, so both paths round identically).  A product that feeds an add
+// or a subtract is fused into one v_fma_f32, as a CUDA build of the
+// reference with default flags (-fmad=true) contracts these expressions.
+__device__ __forceinline__ float upd_hx(float hx, float c, float d) { return fmaf(-c, d, hx); }  // hx - c d
+__device__ __forceinline__ float upd_hy(float hy, float c, float d) { return fmaf(c, d, hy); }   // hy + c d
+// (c0 e + c1 dhy) - c1 dhx
+__device__ __forceinline__ float upd_e(float c0, float c1, float e, float dhy, float dhx) {
+    return fmaf(-c1, dhx, fmaf(c1, dhy, c0 * e));
+}
+// CPML: correction factor (RA - 1) dd + RB psi and next psi = RE psi - RF dd
+__device__ __forceinline__ float cpml_t(float ra01, float rb, float dd, float ps) {
+    return fmaf(rb, ps, ra01 * dd);
+}
+__device__ __forceinline__ float cpml_psi(float re, float rf, float dd, float ps) {
+    return fmaf(re, ps, -(rf * dd));
 }
 
 // Time index of this launch: absolute `koff` when no counter is bound,
@@ -227,26 +248,26 @@ k_h_phase(Dev d, int koff, int dir) {
         // Hx: main update for i >= 1 (uH0 == 1), y-CPML on every row.
         if (i >= 1 || yq >= 0) {
             float hx = d.Hx[b];
-            if (i >= 1) hx = hx - uH1 * (ezj - ez);
+            if (i >= 1) hx = upd_hx(hx, uH1, ezj - ez);
             if (yq >= 0) {
                 float* pp = psi_y(d, yk, s, i, yq);
                 const float ps = *pp;
                 const float dd = (ezj - ez) * d.idx;
-                hx = hx - uH4 * (yRA01 * dd + yRB * ps);
-                *pp = yRE * ps - yRF * dd;
+                hx = fmaf(-uH4, cpml_t(yRA01, yRB, dd, ps), hx);
+                *pp = cpml_psi(yRE, yRF, dd, ps);
             }
             d.Hx[b] = hx;
         }
         // Hy: main update for j >= 1, x-CPML on every column.
         if (j >= 1 || xq >= 0) {
             float hy = d.Hy[b];
-            if (j >= 1) hy = hy + uH1 * (ezi - ez);
+            if (j >= 1) hy = upd_hy(hy, uH1, ezi - ez);
             if (xq >= 0) {
                 float* pp = psi_x(d, xk, s, xq, j);
                 const float ps = *pp;
                 const float dd = (ezi - ez) * d.idx;
-                hy = hy + uH4 * (xRA01 * dd + xRB * ps);
-                *pp = xRE * ps - xRF * dd;
+                hy = fmaf(uH4, cpml_t(xRA01, xRB, dd, ps), hy);
+                *pp = cpml_psi(xRE, xRF, dd, ps);
             }
             d.Hy[b] = hy;
         }
@@ -299,7 +320,7 @@ k_e_phase(Dev d, int koff, int dir) {
         const float hx = d.Hx[b];
         const float hyim = (i >= 1) ? d.Hy[b - pitch] : 0.f;
         const float hxjm = (j >= 1) ? d.Hx[b - 1] : 0.f;
-        if (main_upd) ez = (uE0 * ez + uE1 * (hy - hyim)) - uE1 * (hx - hxjm);
+        if (main_upd) ez = upd_e(uE0, uE1, ez, hy - hyim, hx - hxjm);
         // x-CPML corrects Ez through dHy/dx, y-CPML through dHx/dy.
         float xcorr_d = 0.f, ycorr_d = 0.f;
         if (xq >= 0) xcorr_d = (hy - hyim) * d.idx;
@@ -307,31 +328,31 @@ k_e_phase(Dev d, int koff, int dir) {
         if (xk == EX0 && xq >= 0) {
             float* pp = psi_x(d, EX0, s, xq, j);
             const float ps = *pp;
-            ez = ez + uE4 * (xRA01 * xcorr_d + xRB * ps);
-            *pp = xRE * ps - xRF * xcorr_d;
+            ez = fmaf(uE4, cpml_t(xRA01, xRB, xcorr_d, ps), ez);
+            *pp = cpml_psi(xRE, xRF, xcorr_d, ps);
         }
         if (yk == EY0 && yq >= 0) {
             float* pp = psi_y(d, EY0, s, i, yq);
             const float ps = *pp;
-            ez = ez - uE4 * (yRA01 * ycorr_d + yRB * ps);
-            *pp = yRE * ps - yRF * ycorr_d;
+            ez = fmaf(-uE4, cpml_t(yRA01, yRB, ycorr_d, ps), ez);
+            *pp = cpml_psi(yRE, yRF, ycorr_d, ps);
         }
         if (xk == EXM && xq >= 0) {
             float* pp = psi_x(d, EXM, s, xq, j);
             const float ps = *pp;
-            ez = ez + uE4 * (xRA01 * xcorr_d + xRB * ps);
-            *pp = xRE * ps - xRF * xcorr_d;
+            ez = fmaf(uE4, cpml_t(xRA01, xRB, xcorr_d, ps), ez);
+            *pp = cpml_psi(xRE, xRF, xcorr_d, ps);
         }
         if (yk == EYM && yq >= 0) {
             float* pp = psi_y(d, EYM, s, i, yq);
             const float ps = *pp;
-            ez = ez - uE4 * (yRA01 * ycorr_d + yRB * ps);
-            *pp = yRE * ps - yRF * ycorr_d;
+            ez = fmaf(-uE4, cpml_t(yRA01, yRB, ycorr_d, ps), ez);
+            *pp = cpml_psi(yRE, yRF, ycorr_d, ps);
         }
         if (SRC) {
             for (int k = 0; k < d.nsrc; ++k) {
                 const int* sc = d.src + 2 * ((size_t)s * d.nsrc + k);
-                if (sc[0] == i && sc[1] == j) ez = ez - uE4 * d.srcscale[(size_t)k * d.nt + n];
+                if (sc[0] == i && sc[1] == j) ez = fmaf(-uE4, d.srcscale[(size_t)k * d.nt + n], ez);
             }
         }
         d.Ez[b] = ez;
@@ -658,16 +679,6 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (gj >= 1 && gj <= P) yqe = P - gj;
         else if (gj >= NJ - P) { yqe = gj - (NJ - P); ye_lo = false; }
     }
-    // y-slab coefficients of this column (RA - 1, RB, RE, RF for H and E)
-    float yh[4] = {0.f, 0.f, 0.f, 0.f}, ye[4] = {0.f, 0.f, 0.f, 0.f};
-    if (yqh >= 0) {
-        yh[0] = d.pml[4 * P + yqh] - 1.0f; yh[1] = d.pml[5 * P + yqh];
-        yh[2] = d.pml[6 * P + yqh]; yh[3] = d.pml[7 * P + yqh];
-    }
-    if (yqe >= 0) {
-        ye[0] = d.pml[yqe] - 1.0f; ye[1] = d.pml[P + yqe];
-        ye[2] = d.pml[2 * P + yqe]; ye[3] = d.pml[3 * P + yqe];
-    }
 
     float ez[CPT], hx[CPT], hy[CPT];
     float cE0[CPT], cE1[CPT], cE4[CPT], cH1[CPT], cH4[CPT];
@@ -683,38 +694,16 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const float* pin = a.psi_in;
     const size_t xs = xslab_floats(d), ys = yslab_floats(d);
 
-    // x-slab coefficients of each owned row (RA - 1, RB, RE, RF for H and E):
-    // row-uniform, so they sit in scalar registers for the whole launch.
-    auto uni = [](float v) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-    };
-    // (The adjoint, which holds more registers, reads them from an LDS table.)
-    __shared__ float sP[ADJ ? 8 * kPmax : 1];
-    if (ADJ)
-        for (int e = threadIdx.x; e < 8 * P; e += NT) sP[e] = d.pml[e];
-    float xH[CPT][4], xE[CPT][4];
-#pragma unroll
-    for (int m = 0; m < CPT && !ADJ; ++m) {
-        const int gi = gi0 + r0 + m;
-        const bool in = P > 0 && gi >= 0 && gi < NI;
-        const int qh = in ? xq_h(gi) : -1;
-        const int qe = in ? xq_e(gi) : -1;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            xH[m][c] = qh >= 0 ? uni(d.pml[(4 + c) * P + qh]) : 0.f;
-            xE[m][c] = qe >= 0 ? uni(d.pml[c * P + qe]) : 0.f;
-        }
-        xH[m][0] = xH[m][0] - 1.0f;
-        xE[m][0] = xE[m][0] - 1.0f;
+    // CPML coefficients (RA - 1, RB, RE, RF) by slab index, H at [0, P), E at
+    // [kPmax, kPmax + P), for both slab directions: one 16-byte LDS read per
+    // CPML term (a broadcast for the row-uniform x-slab terms).
+    __shared__ float4 sXC[2 * kPmax];
+    for (int e = threadIdx.x; e < 2 * P; e += NT) {
+        const bool h = e < P;
+        const int q = h ? e : e - P;
+        const float* b = d.pml + (h ? 4 * P : 0) + q;
+        sXC[(h ? 0 : kPmax) + q] = make_float4(b[0] - 1.0f, b[P], b[2 * P], b[3 * P]);
     }
-    auto XH = [&](int m, int q, int c) -> float {
-        if constexpr (ADJ) return c == 0 ? sP[4 * P + q] - 1.0f : sP[(4 + c) * P + q];
-        else return xH[m][c];
-    };
-    auto XE = [&](int m, int q, int c) -> float {
-        if constexpr (ADJ) return c == 0 ? sP[q] - 1.0f : sP[c * P + q];
-        else return xE[m][c];
-    };
 
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
@@ -785,19 +774,23 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     for (int m = 0; m < CPT; ++m) { ae[m] = 0.0; as_[m] = 0.0; }
     __syncthreads();
 
-    // Row-uniform facts of each owned row: gi >= 1, x-slab indices for H and
-    // E (-1 outside the slabs and outside the domain), minus-side flag.
-    bool rge1[CPT], xminus[CPT];
+    // Row-uniform facts of each owned row, in scalar registers: bit m of rfl
+    // = (gi >= 1), bit 16 + m = (gi <= P, the minus side); x-slab indices for
+    // H and E (-1 outside the slabs and outside the domain).
+    unsigned rfl = 0u;
     int qh_m[CPT], qe_m[CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int gi = gi0 + r0 + m;
         const bool in = P > 0 && gi >= 0 && gi < NI;
-        rge1[m] = gi >= 1;
-        qh_m[m] = in ? xq_h(gi) : -1;
-        qe_m[m] = in ? xq_e(gi) : -1;
-        xminus[m] = gi <= P;
+        rfl |= (gi >= 1 ? 1u : 0u) << m;
+        rfl |= (gi <= P ? 1u : 0u) << (16 + m);
+        qh_m[m] = __builtin_amdgcn_readfirstlane(in ? xq_h(gi) : -1);
+        qe_m[m] = __builtin_amdgcn_readfirstlane(in ? xq_e(gi) : -1);
     }
+    rfl = __builtin_amdgcn_readfirstlane(rfl);
+    auto rge1 = [&](int m) -> bool { return (rfl >> m) & 1u; };
+    auto xminus = [&](int m) -> bool { return (rfl >> (16 + m)) & 1u; };
 
     // General H half-step.  Cells outside the domain (and the padding row and
     // column) have zero coefficients and zero fields, so the plain update
@@ -809,25 +802,25 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
             const float e = ez[m];
-            const float ezj = lane_next(e);
-            const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below;
+            const float dj = lane_next(e) - e;
+            const float di = ((m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below) - e;
             const int o = (r0 + m) * 64 + lane;
-            float vx = hx[m];
-            if (rge1[m]) vx = vx - cH1[m] * (ezj - e);
-            float vy = hy[m] + cH1[m] * (ezi - e);
-            vy = (gj >= 1) ? vy : hy[m];
+            float vx = rge1(m) ? upd_hx(hx[m], cH1[m], dj) : hx[m];
+            float vy = (gj >= 1) ? upd_hy(hy[m], cH1[m], di) : hy[m];
             if (yqh >= 0) {
-                const float dd = (ezj - e) * idx;
+                const float4 yc = sXC[yqh];
+                const float dd = dj * idx;
                 const float ph = sPsi[2][o];
-                vx = vx - cH4[m] * (yh[0] * dd + yh[1] * ph);
-                sPsi[2][o] = yh[2] * ph - yh[3] * dd;
+                vx = fmaf(-cH4[m], cpml_t(yc.x, yc.y, dd, ph), vx);
+                sPsi[2][o] = cpml_psi(yc.z, yc.w, dd, ph);
             }
             const int q = qh_m[m];
             if (q >= 0) {
-                const float dd = (ezi - e) * idx;
+                const float4 xc = sXC[q];
+                const float dd = di * idx;
                 const float ph = sPsi[0][o];
-                vy = vy + cH4[m] * (XH(m, q, 0) * dd + XH(m, q, 1) * ph);
-                sPsi[0][o] = XH(m, q, 2) * ph - XH(m, q, 3) * dd;
+                vy = fmaf(cH4[m], cpml_t(xc.x, xc.y, dd, ph), vy);
+                sPsi[0][o] = cpml_psi(xc.z, xc.w, dd, ph);
             }
             hx[m] = vx;
             hy[m] = vy;
@@ -839,27 +832,30 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     // exceptions, then the CPML terms in slab order x0, y0, xmax, ymax.
     auto e_cell = [&](int m, int o, float hyim, float hxjm) -> float {
         const float e0 = ez[m];
-        float v = (cE0[m] * e0 + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
-        if (!rge1[m]) v = e0;
+        const float dhy = hy[m] - hyim, dhx = hx[m] - hxjm;
+        float v = upd_e(cE0[m], cE1[m], e0, dhy, dhx);
+        if (!rge1(m)) v = e0;
         v = (gj >= 1) ? v : e0;
         const float c4 = cE4[m];
         auto yterm = [&]() {
-            const float dyd = (hx[m] - hxjm) * idx;
+            const float4 yc = sXC[kPmax + yqe];
+            const float dyd = dhx * idx;
             const float p = sPsi[3][o];
-            v = v - c4 * (ye[0] * dyd + ye[1] * p);
-            sPsi[3][o] = ye[2] * p - ye[3] * dyd;
+            v = fmaf(-c4, cpml_t(yc.x, yc.y, dyd, p), v);
+            sPsi[3][o] = cpml_psi(yc.z, yc.w, dyd, p);
         };
         const int q = qe_m[m];
         if (q < 0) {
             if (yqe >= 0) yterm();
         } else {
-            const float dxd = (hy[m] - hyim) * idx;
+            const float dxd = dhy * idx;
             auto xterm = [&]() {
+                const float4 xc = sXC[kPmax + q];
                 const float p = sPsi[1][o];
-                v = v + c4 * (XE(m, q, 0) * dxd + XE(m, q, 1) * p);
-                sPsi[1][o] = XE(m, q, 2) * p - XE(m, q, 3) * dxd;
+                v = fmaf(c4, cpml_t(xc.x, xc.y, dxd, p), v);
+                sPsi[1][o] = cpml_psi(xc.z, xc.w, dxd, p);
             };
-            if (xminus[m]) {
+            if (xminus(m)) {
                 xterm();
                 if (yqe >= 0) yterm();
             } else {
@@ -882,8 +878,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             const float e = ez[m];
             const float ezj = lane_next(e);
             const float ezi = (m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below;
-            hx[m] = hx[m] - cH1[m] * (ezj - e);
-            hy[m] = hy[m] + cH1[m] * (ezi - e);
+            hx[m] = upd_hx(hx[m], cH1[m], ezj - e);
+            hy[m] = upd_hy(hy[m], cH1[m], ezi - e);
         }
         sHyBot[w * 64 + lane] = hy[CPT - 1];
     };
@@ -988,7 +984,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                     float v;
                     if (FAST) {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
-                        v = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+                        v = upd_e(cE0[m], cE1[m], ez[m], hy[m] - hyim, hx[m] - hxjm);
                     } else {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
                         v = e_cell(m, r * 64 + lane, hyim, hxjm);
@@ -996,7 +992,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                     if (src_any && (srcbits & (1u << m))) {
                         for (int kk = 0; kk < d.nsrc; ++kk) {
                             const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
-                            if (sc[0] == gi && sc[1] == gj) v = v - cE4[m] * d.srcscale[(size_t)kk * d.nt + n];
+                            if (sc[0] == gi && sc[1] == gj) v = fmaf(-cE4[m], d.srcscale[(size_t)kk * d.nt + n], v);
                         }
                     }
                     ez[m] = v;
@@ -1034,7 +1030,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                     float lam;
                     if (FAST) {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
-                        lam = (cE0[m] * ez[m] + cE1[m] * (hy[m] - hyim)) - cE1[m] * (hx[m] - hxjm);
+                        lam = upd_e(cE0[m], cE1[m], ez[m], hy[m] - hyim, hx[m] - hxjm);
                     } else {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
                         lam = e_cell(m, r * 64 + lane, hyim, hxjm);
