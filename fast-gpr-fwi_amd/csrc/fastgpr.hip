@@ -613,18 +613,50 @@ struct TBArgs {
     float* Hy_out;
     float* psi_out;
     int ti_count, tj_count;
+    int int_ti0, int_ni, int_tj0, int_nj;   // rectangle of interior tiles (no CPML cell, no edge)
     const int* rx_ptr;   // CSR over S*ntiles: receivers sampled (forward) / injected (adjoint)
     const int* rx_idx;   // entries s*nrx + r
     double* gshot;       // adjoint: 2*S*NI*NJ per-shot partial gradients
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical ids so that neighbouring tiles (and the shots
-// of one tile, which share coefficients) meet in the same L2.
-__device__ __forceinline__ int xcd_remap(int b, int nwg) {
-    const int q = nwg / 8, r = nwg % 8;
-    const int x = b % 8, t = b / 8;
-    return x * q + (x < r ? x : r) + t;
+// Work order of a launch.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b to XCD b % 8, in order b / 8 within it).  Each XCD gets an
+// equal share of the interior items first and then an equal share of the
+// general (CPML / edge) items, which cost about 1.6x as much: the cheap ones
+// finish early and free their CUs for the expensive ones, instead of an
+// expensive tail.  An item is (tile, shot), shots innermost, so the shots of
+// a tile (which share coefficients) meet in one XCD's L2.  Returns the
+// shot and the tile of block `wg` (a bijection over [0, nwg)).
+__device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg, int& s, int& tile) {
+    const int tjc = a.tj_count;
+    const int ni = a.int_ni, nj = a.int_nj;
+    const int Ni = ni * nj * S;                      // interior items
+    const int x = wg & 7, t = wg >> 3;
+    const int q = nwg >> 3, r = nwg & 7;
+    const int qi = Ni >> 3, ri = Ni & 7;
+    const int base = x * q + (x < r ? x : r);        // items of XCDs < x
+    const int ia = x * qi + (x < ri ? x : ri);       // interior items of XCDs < x
+    const int icnt = qi + (x < ri ? 1 : 0);
+    if (t < icnt) {
+        const int k = ia + t, kt = k / S;
+        s = k - kt * S;
+        const int ti = a.int_ti0 + kt / nj;
+        tile = ti * tjc + a.int_tj0 + (kt - (kt / nj) * nj);
+        return;
+    }
+    const int g = (base - ia) + (t - icnt), gt = g / S;
+    s = g - gt * S;
+    // general tiles in row-major order with the interior rectangle skipped
+    const int before = a.int_ti0 * tjc;              // rows above the rectangle
+    const int per = tjc - nj;                        // general tiles of a rectangle row
+    if (ni == 0 || gt < before) { tile = gt; return; }
+    const int u = gt - before;
+    if (u < ni * per) {
+        const int ti = a.int_ti0 + u / per, c = u - (u / per) * per;
+        tile = ti * tjc + (c < a.int_tj0 ? c : c + nj);
+        return;
+    }
+    tile = (a.int_ti0 + ni) * tjc + (u - ni * per);
 }
 
 __device__ __forceinline__ float lane_next(float v) {   // value of lane + 1 (0 on lane 63)
@@ -648,9 +680,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const int NI = d.NI, NJ = d.NJ, P = d.P, pitch = d.pitch;
 
     const int ntiles = a.ti_count * a.tj_count;
-    const int L = xcd_remap(wg, nwg);
-    const int s = L % d.S;
-    const int tile = L / d.S;
+    int s, tile;
+    tb_item(a, d.S, wg, nwg, s, tile);
     const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
     const int gi0 = ti * TI - K, gj0 = tj * TJ - K;
     const int lane = threadIdx.x & 63;
@@ -1286,6 +1317,19 @@ TBArgs make_tb(const fgpr_problem& p, const Dev& d, int parity, bool adj) {
     a.psi_out = parity ? p.psi : p.psi2;
     a.ti_count = p.tb_ti;
     a.tj_count = p.tb_tj;
+    // Interior tiles: gi0 = ti TI - K with gi0 >= P + 1 and gi0 + RI <= NI - P
+    // (and the same for columns), a rectangle of tile indices.
+    {
+        const int TI = p.tb_ri - 2 * p.tb_k - 1, TJ = kTBCols - 2 * p.tb_k - 1;
+        auto range = [](int lo_num, int hi_num, int T, int count, int& first, int& n) {
+            first = (lo_num + T - 1) / T;                       // lo_num >= 1
+            const int last = hi_num < 0 ? -1 : (hi_num / T < count - 1 ? hi_num / T : count - 1);
+            n = last >= first ? last - first + 1 : 0;
+        };
+        range(p.P + 1 + p.tb_k, p.NI - p.P - p.tb_ri + p.tb_k, TI, p.tb_ti, a.int_ti0, a.int_ni);
+        range(p.P + 1 + p.tb_k, p.NJ - p.P - kTBCols + p.tb_k, TJ, p.tb_tj, a.int_tj0, a.int_nj);
+        if (a.int_ni == 0 || a.int_nj == 0) { a.int_ti0 = 0; a.int_ni = 0; a.int_tj0 = 0; a.int_nj = 0; }
+    }
     a.rx_ptr = adj ? p.tb_inj_ptr : p.tb_rx_ptr;
     a.rx_idx = adj ? p.tb_inj_idx : p.tb_rx_idx;
     a.gshot = p.gshot;
