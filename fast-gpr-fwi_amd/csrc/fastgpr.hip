@@ -659,6 +659,11 @@ __device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg,
     tile = (a.int_ti0 + ni) * tjc + (u - ni * per);
 }
 
+// Load through the scalar cache (a wave-uniform address of read-only data).
+__device__ __forceinline__ int sload(const int* p) {
+    return *(const __attribute__((address_space(4))) int*)p;
+}
+
 __device__ __forceinline__ float lane_next(float v) {   // value of lane + 1 (0 on lane 63)
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
                                                                   0x130, 0xf, 0xf, false));
@@ -725,21 +730,26 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const float* pin = a.psi_in;
     const size_t xs = xslab_floats(d), ys = yslab_floats(d);
 
-    // CPML coefficients (RA - 1, RB, RE, RF) by slab index, H at [0, P), E at
-    // [kPmax, kPmax + P), for both slab directions: one 16-byte LDS read per
-    // CPML term (a broadcast for the row-uniform x-slab terms).
-    __shared__ float4 sXC[2 * kPmax];
-    for (int e = threadIdx.x; e < 2 * P; e += NT) {
-        const bool h = e < P;
-        const int q = h ? e : e - P;
-        const float* b = d.pml + (h ? 4 * P : 0) + q;
-        sXC[(h ? 0 : kPmax) + q] = make_float4(b[0] - 1.0f, b[P], b[2 * P], b[3 * P]);
+    // Set-up issues every independent load before the first wait: the
+    // receiver list bounds (scalar loads) and its first 64 entries, the
+    // region's fields, coefficients and CPML memory, then the dependent
+    // receiver positions and the CPML coefficient table.
+    const int lst = a.rx_ptr ? sload(a.rx_ptr + s * ntiles + tile) : 0;
+    const int lcount = (a.rx_ptr ? sload(a.rx_ptr + s * ntiles + tile + 1) : 0) - lst;
+    const int t_first = lane < lcount ? a.rx_idx[lst + lane] : -1;
+    // CPML coefficient table entry of this thread (threads [0, 2P)), stored
+    // to LDS once every load is in
+    float4 xcv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((int)threadIdx.x < 2 * P) {
+        const bool h = (int)threadIdx.x < P;
+        const float* b = d.pml + (h ? 4 * P : -P) + (int)threadIdx.x;
+        xcv = make_float4(b[0] - 1.0f, b[P], b[2 * P], b[3 * P]);
     }
 
+    float ps[4][CPT];
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int r = r0 + m, gi = gi0 + r;
-        const int o = r * 64 + lane;
         const bool ex = colx && gi >= 0 && gi <= NI;
         const size_t g = shot_off + (size_t)gi * pitch + gj;
         ez[m] = ex ? a.Ez_in[g] : 0.f;
@@ -755,25 +765,22 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (ac) act |= 1u << m;
         const int qh = (ac && P > 0) ? xq_h(gi) : -1;
         const int qe = (ac && P > 0) ? xq_e(gi) : -1;
-        sPsi[0][o] = qh >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] : 0.f;
-        sPsi[1][o] = qe >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] : 0.f;
-        sPsi[2][o] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
-        sPsi[3][o] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
+        ps[0][m] = qh >= 0 ? pin[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] : 0.f;
+        ps[1][m] = qe >= 0 ? pin[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] : 0.f;
+        ps[2][m] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
+        ps[3][m] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
     }
-    sEzTop[w * 64 + lane] = ez[0];
-    sHyBot[w * 64 + lane] = hy[CPT - 1];
 
     // This wave's receivers (forward: tile-centre ones; adjoint: region
     // ones), compacted in list order with a ballot per 64 entries.
-    const int lst = a.rx_ptr ? a.rx_ptr[s * ntiles + tile] : 0;
-    const int lcount = (a.rx_ptr ? a.rx_ptr[s * ntiles + tile + 1] : 0) - lst;
     int wn = 0;
     for (int base = 0; base < lcount; base += 64) {
         const int e = base + lane;
+        const int t = base == 0 ? t_first : (e < lcount ? a.rx_idx[lst + e] : -1);
         int ent = -1;
-        if (e < lcount) {
-            const int t = a.rx_idx[lst + e];
-            const int li = d.rx[2 * t] - gi0, lj = d.rx[2 * t + 1] - gj0;
+        if (t >= 0) {
+            const int2 pos = *(const int2*)(d.rx + 2 * t);
+            const int li = pos.x - gi0, lj = pos.y - gj0;
             if (li >= r0 && li < r0 + CPT && lj >= 0 && lj < 64)
                 ent = ((li - r0) << 26) | (lj << 20) | (t - (t / d.nrx) * d.nrx);
         }
@@ -783,6 +790,22 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         wn += (int)__popcll(bal);
     }
     wn = wn < kWaveList ? wn : kWaveList;
+
+    // CPML coefficients (RA - 1, RB, RE, RF) by slab index, H at [0, P), E at
+    // [kPmax, kPmax + P), for both slab directions: one 16-byte LDS read per
+    // CPML term (a broadcast for the row-uniform x-slab terms).
+    __shared__ float4 sXC[2 * kPmax];
+    if ((int)threadIdx.x < 2 * P)
+        sXC[(int)threadIdx.x < P ? (int)threadIdx.x : kPmax - P + (int)threadIdx.x] = xcv;
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+        const int o = (r0 + m) * 64 + lane;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sPsi[c][o] = ps[c][m];
+    }
+    sEzTop[w * 64 + lane] = ez[0];
+    sHyBot[w * 64 + lane] = hy[CPT - 1];
+
     // one entry per lane, kWaveList / 64 registers
     int rxent[kWaveList / 64];
 #pragma unroll
@@ -1102,6 +1125,19 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 
     // ---- write back the tile centre ----
     float* pout = a.psi_out;
+    // adjoint: read every partial-gradient pair first (one round trip), add
+    // after the field stores are issued
+    double g0v[ADJ ? CPT : 1], g1v[ADJ ? CPT : 1];
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int r = r0 + m, gi = gi0 + r;
+            const bool own = r >= K && r < K + TI && centre_col && (act & (1u << m));
+            const double* gp = a.gshot + ((size_t)s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
+            g0v[m] = own ? gp[0] : 0.0;
+            g1v[m] = own ? gp[(size_t)d.S * NI * NJ] : 0.0;
+        }
+    }
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int r = r0 + m, gi = gi0 + r;
@@ -1119,11 +1155,10 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (qe >= 0) pout[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] = sPsi[1][o];
         if (yqh >= 0) pout[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] = sPsi[2][o];
         if (yqe >= 0) pout[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] = sPsi[3][o];
-        if (ADJ) {
+        if constexpr (ADJ) {
             double* g0 = a.gshot + ((size_t)s * NI + gi) * NJ + gj;
-            double* g1 = g0 + (size_t)d.S * NI * NJ;
-            *g0 += ae[m];
-            *g1 += as_[m];
+            g0[0] = g0v[m] + ae[m];
+            g0[(size_t)d.S * NI * NJ] = g1v[m] + as_[m];
         }
     }
 }
