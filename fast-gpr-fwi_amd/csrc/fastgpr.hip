@@ -616,7 +616,7 @@ struct TBArgs {
     int int_ti0, int_ni, int_tj0, int_nj;   // rectangle of interior tiles (no CPML cell, no edge)
     const int* rx_ptr;   // CSR over S*ntiles: receivers sampled (forward) / injected (adjoint)
     const int* rx_idx;   // entries s*nrx + r
-    double* gshot;       // adjoint: 2*S*NI*NJ per-shot partial gradients
+    double* gshot;       // adjoint: S x 2 x NI x NJ per-shot partial gradients (shot-major)
 };
 
 // Work order of a launch.  Workgroups are dispatched round-robin over the 8
@@ -1133,9 +1133,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         for (int m = 0; m < CPT; ++m) {
             const int r = r0 + m, gi = gi0 + r;
             const bool own = r >= K && r < K + TI && centre_col && (act & (1u << m));
-            const double* gp = a.gshot + ((size_t)s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
+            const double* gp = a.gshot + ((size_t)2 * s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
             g0v[m] = own ? gp[0] : 0.0;
-            g1v[m] = own ? gp[(size_t)d.S * NI * NJ] : 0.0;
+            g1v[m] = own ? gp[(size_t)NI * NJ] : 0.0;
         }
     }
 #pragma unroll
@@ -1156,9 +1156,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (yqh >= 0) pout[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] = sPsi[2][o];
         if (yqe >= 0) pout[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] = sPsi[3][o];
         if constexpr (ADJ) {
-            double* g0 = a.gshot + ((size_t)s * NI + gi) * NJ + gj;
+            double* g0 = a.gshot + ((size_t)2 * s * NI + gi) * NJ + gj;
             g0[0] = g0v[m] + ae[m];
-            g0[(size_t)d.S * NI * NJ] = g1v[m] + as_[m];
+            g0[(size_t)NI * NJ] = g1v[m] + as_[m];
         }
     }
 }
@@ -1220,15 +1220,15 @@ __global__ void __launch_bounds__(256) k_snap_pack(Dev d, int RI, int NT, int K,
         in ? Ez[((int64_t)sh * (d.NI + 1) + gi) * d.pitch + gj] : 0.f;
 }
 
-// grad[c] += sum_s gshot[p][s][c] in shot order (deterministic).
+// grad[c] += sum_s gshot[s][p][c] in shot order (deterministic).
 __global__ void k_reduce_shots(const double* __restrict__ gshot, int S, int64_t ncell,
                                double* grad_eps, double* grad_sig) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     double se = 0.0, ss = 0.0;
     for (int s = 0; s < S; ++s) {
-        se += gshot[(size_t)s * ncell + c];
-        ss += gshot[((size_t)S + s) * ncell + c];
+        se += gshot[(size_t)2 * s * ncell + c];
+        ss += gshot[((size_t)2 * s + 1) * ncell + c];
     }
     grad_eps[c] += se;
     grad_sig[c] += ss;
@@ -1755,7 +1755,12 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
         return fail(FGPR_EINVAL, "unknown gradient mode");
     const bool exact = grad_mode == FGPR_GRAD_EXACT;
     if (n_lo < (exact ? 0 : 1) || n_hi >= p->nt || n_hi < n_lo - 1) return fail(FGPR_EINVAL, "bad step range");
-    if (!p->snaps || !p->grad_eps || !p->grad_sig) return fail(FGPR_EINVAL, "adjoint needs snaps and gradients");
+    // The time-blocked reference adjoint may leave the shot reduction to the
+    // caller (both gradient pointers null): several shot groups of one batch
+    // then run on separate streams and fgpr_reduce_shots sums them in order.
+    const bool defer = !exact && tb_enabled(*p) && !p->grad_eps && !p->grad_sig;
+    if (!p->snaps || (!defer && (!p->grad_eps || !p->grad_sig)))
+        return fail(FGPR_EINVAL, "adjoint needs snaps and gradients");
     if (p->nrx > 0 && (!p->resid || !p->rx)) return fail(FGPR_EINVAL, "null residual/receivers");
     if (p->snap_mod > 0) {
         if (p->nsnap < p->snap_mod || n_hi - (n_lo >= 1 ? n_lo - 1 : 0) + 1 > p->snap_mod)
@@ -1808,6 +1813,7 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
             n -= ns;
         }
         if ((rc = tb_settle(q, parity, st))) return rc;
+        if (defer) return FGPR_OK;
         const int64_t ncell = (int64_t)p->NI * p->NJ;
         hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
                            p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);

@@ -27,7 +27,8 @@ _cache: dict = {}
 
 def _key(G) -> str:
     name = torch.cuda.get_device_name(G.device)
-    return f"{name}|{G.NI}x{G.NJ}|P{G.pmlthick}|S{G.S}|rx{G.nrx}"
+    groups = max(1, min(int(G.options.get("shot_streams", 3) or 1), G.S))
+    return f"{name}|{G.NI}x{G.NJ}|P{G.pmlthick}|S{G.S}|rx{G.nrx}|g{groups}"
 
 
 def _load_file():

@@ -22,6 +22,12 @@ Extra keyword arguments (all default to the reference behaviour):
                      recompute snapshots segment by segment in the adjoint),
                      or "auto" (checkpoint only when the history would not
                      fit in half the free device memory).
+* ``shot_streams`` -- shot groups of a time-blocked sweep, each on its own
+                     HIP stream so that the launches of one fill the CUs the
+                     other's launch tail leaves idle (default 3; 1 = one
+                     batch on the current stream).  Traces are bitwise those
+                     of one batch; the gradient is reduced over all shots in
+                     shot order either way.
 """
 from __future__ import annotations
 
@@ -63,7 +69,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
             exec_mode="graph", graph_chunk=32, time_block=None, gradient="reference",
-            checkpoint=None):
+            checkpoint=None, shot_streams=3):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
     Same arguments and meaning as compute.py:33-50: ``nx`` is the depth
@@ -78,7 +84,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
                        source_apmlitudes, step, er, se, mr, nsrc, nrx, dt, pml_formulation,
                        freq, total_step, tv, Fixedreceiver, nseg, customsource, customreceiver,
                        exec_mode=exec_mode, graph_chunk=graph_chunk, time_block=time_block,
-                       gradient=gradient, checkpoint=checkpoint)
+                       gradient=gradient, checkpoint=checkpoint, shot_streams=shot_streams)
     if G.time_block == "auto":
         from .autotune import choose
         G.time_block = choose(G, er, se, mr)
@@ -91,7 +97,7 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                    mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
                    nseg, customsource=None, customreceiver=None, exec_mode="graph",
                    graph_chunk=32, time_block=None, gradient="reference",
-                   checkpoint=None) -> FDTDGrid:
+                   checkpoint=None, shot_streams=3) -> FDTDGrid:
     """Build the solve description (compute.py:726-872).
 
     ``nseg`` (the reference's unfinished segmenting, compute.py:543-553) and
@@ -109,6 +115,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
         raise ValueError("exec_mode must be 'graph' or 'eager'")
     if gradient not in ("reference", "exact"):
         raise ValueError("gradient must be 'reference' or 'exact'")
+    if int(shot_streams) < 1:
+        raise ValueError("shot_streams must be >= 1")
     NI, NJ = int(nx), int(ny)
     if int(nz) != 1:
         raise ValueError("2-D TMz solver: nz must be 1")
@@ -135,7 +143,8 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                     srcwaves=srcwaves, pml=pml, tv=tv, exec_mode=exec_mode,
                     graph_chunk=int(graph_chunk),
                     time_block=time_block if time_block in (None, "auto") else tuple(time_block),
-                    options={"gradient": gradient, "checkpoint": checkpoint})
+                    options={"gradient": gradient, "checkpoint": checkpoint,
+                             "shot_streams": int(shot_streams)})
 
 
 class GPRFWI(torch.autograd.Function):

@@ -101,12 +101,15 @@ class Batch:
     """Device buffers of one batch of S shots on one grid.
 
     ``tb`` = (region rows, threads, steps per launch) selects the
-    time-blocked kernels; None selects the per-step kernels."""
+    time-blocked kernels; None selects the per-step kernels.  ``shots`` =
+    (lo, hi) restricts the batch to shots lo..hi-1 of ``G`` (a shot group);
+    ``traces`` may then be the group's slice of a whole-batch tensor."""
 
-    def __init__(self, G, store: bool = True, tb=None, tiled=None):
+    def __init__(self, G, store: bool = True, tb=None, tiled=None, shots=None, traces=None):
         dev = G.device
         self.G = G
-        self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, G.S, G.iterations
+        self.lo, self.hi = shots if shots is not None else (0, G.S)
+        self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, self.hi - self.lo, G.iterations
         self.P = G.pmlthick
         self.pitch = _native.pitch(self.NJ)
         self.fields = torch.zeros((3, self.S, self.NI + 1, self.pitch), dtype=torch.float32, device=dev)
@@ -115,14 +118,15 @@ class Batch:
         self.pml = (G.pml.to(dev).contiguous() if self.P > 0
                     else torch.zeros(8, dtype=torch.float32, device=dev))
         from .grid import device_cells
-        if G.source.shape[0] < self.S or G.receiver.shape[0] < self.S:
-            raise ValueError(f"source/receiver arrays have fewer rows than the {self.S} shots")
-        self.src = device_cells(G.source[: self.S], self.NI, self.NJ, "source").to(dev)
-        self.rx = device_cells(G.receiver[: self.S], self.NI, self.NJ, "receiver").to(dev)
+        if G.source.shape[0] < self.hi or G.receiver.shape[0] < self.hi:
+            raise ValueError(f"source/receiver arrays have fewer rows than the {self.hi} shots")
+        self.src = device_cells(G.source[self.lo:self.hi], self.NI, self.NJ, "source").to(dev)
+        self.rx = device_cells(G.receiver[self.lo:self.hi], self.NI, self.NJ, "receiver").to(dev)
         self.nsrc = self.src.shape[1]
         self.nrx = self.rx.shape[1]
         self.ctr = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.traces = torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev)
+        self.traces = (traces if traces is not None else
+                       torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev))
         self.tb = tuple(tb) if tb else None
         # Snapshots in the time-blocked layout when both sweeps run the same
         # time-blocked variant (the exact adjoint reads the row-major one).
@@ -164,9 +168,10 @@ class Batch:
             self.psi2.zero_()
 
     def gshot_buffer(self):
-        """Per-shot FP64 partial gradients of the time-blocked adjoint (zeroed)."""
+        """Per-shot FP64 partial gradients of the time-blocked adjoint,
+        (S, 2, NI, NJ), zeroed."""
         if self.gshot is None:
-            self.gshot = torch.zeros((2, self.S, self.NI, self.NJ), dtype=torch.float64,
+            self.gshot = torch.zeros((self.S, 2, self.NI, self.NJ), dtype=torch.float64,
                                      device=self.fields.device)
         else:
             self.gshot.zero_()
@@ -209,6 +214,79 @@ class Batch:
         return p
 
 
+class ShotGroups:
+    """One batch split into shot groups whose sweeps run on separate HIP
+    streams.  A time-blocked launch of one group finishes its last, partial
+    round of workgroups on part of the CUs; the other group's launches run on
+    the rest, so the tails of the launches overlap instead of idling CUs.
+    Each group keeps its own state; traces and the per-shot partial
+    gradients are slices of whole-batch tensors, and the gradient is reduced
+    once over all shots in shot order, as for one batch."""
+
+    def __init__(self, G, store: bool, tb, n: int):
+        dev = G.device
+        self.G, self.S, self.nt, self.NI, self.NJ, self.tb = G, G.S, G.iterations, G.NI, G.NJ, tb
+        cuts = [round(k * G.S / n) for k in range(n + 1)]
+        self.nrx = int(G.receiver.shape[1])
+        self.traces = torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev)
+        self.groups = [Batch(G, store=store, tb=tb, shots=(cuts[k], cuts[k + 1]),
+                             traces=self.traces[cuts[k]:cuts[k + 1]]) for k in range(n)]
+        self.gshot = None
+
+    def reset_state(self):
+        for g in self.groups:
+            g.reset_state()
+
+    def gshot_buffer(self):
+        if self.gshot is None:
+            self.gshot = torch.zeros((self.S, 2, self.NI, self.NJ), dtype=torch.float64,
+                                     device=self.traces.device)
+        else:
+            self.gshot.zero_()
+        for g in self.groups:
+            g.gshot = self.gshot[g.lo:g.hi]
+        return self.gshot
+
+
+_SIDE_STREAMS: dict = {}
+
+
+def _side_streams(device, n: int) -> list:
+    """Per-device side streams (default priority) for shot groups."""
+    key = torch.device(device).index or 0
+    have = _SIDE_STREAMS.setdefault(key, [])
+    while len(have) < n:
+        have.append(torch.cuda.Stream(device))
+    return have[:n]
+
+
+def _fan_out(device, groups, run):
+    """run(group, stream) for every group on its own side stream, ordered
+    after the work queued so far on the current stream, which then waits for
+    all of them."""
+    main = torch.cuda.current_stream(device)
+    start = torch.cuda.Event()
+    start.record(main)
+    streams = _side_streams(device, len(groups))
+    for g, st in zip(groups, streams):
+        st.wait_event(start)
+        with torch.cuda.stream(st):
+            run(g, ctypes.c_void_p(st.cuda_stream))
+    for st in streams:
+        done = torch.cuda.Event()
+        done.record(st)
+        main.wait_event(done)
+
+
+def shot_groups(G) -> int:
+    """Shot groups of a full-history time-blocked sweep (option
+    ``shot_streams``, default 3; 1 = one batch on the current stream)."""
+    n = int(G.options.get("shot_streams", 3) or 1)
+    if not G.time_block or G.options.get("gradient", "reference") == "exact":
+        return 1
+    return max(1, min(n, G.S))
+
+
 def source_scale(G) -> torch.Tensor:
     """w[n]*dl/(dx*dy*dz) per source row in float32, the expression of
     sourcereceiver.cu:73-75 with dl = dy = dz = dx (compute.py:571, 693)."""
@@ -237,6 +315,16 @@ def forward(G, er, se, mr, store: bool = True):
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
     coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
+    n = shot_groups(G)
+    if n > 1:
+        b = ShotGroups(G, store, G.time_block, n)
+
+        def run(g, st):
+            p = g.problem(coef, srcscale=emitting_sources(G, g))
+            check(lib.fgpr_forward(ctypes.byref(p), 0, g.nt, _MODES[G.exec_mode], G.graph_chunk, st),
+                  "forward")
+        _fan_out(dev, b.groups, run)
+        return b.traces, b
     b = Batch(G, store=store, tb=G.time_block)
     scale = emitting_sources(G, b)
     p = b.problem(coef, srcscale=scale)
@@ -302,6 +390,21 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     NI, NJ = b.NI, b.NJ
     ge = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
     gs = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
+    if isinstance(b, ShotGroups):
+        b.gshot_buffer()
+
+        def run(g, st):
+            # the shot reduction is deferred (null gradients) to one pass below
+            p = g.problem(coef, srcscale=getattr(g, "srcscale", None), resid=resid[g.lo:g.hi])
+            check(lib.fgpr_adjoint(ctypes.byref(p), g.nt - 1, 1, _native.GRAD_REFERENCE,
+                                   _MODES[G.exec_mode], G.graph_chunk, st), "adjoint")
+        if b.nt >= 2:
+            _fan_out(dev, b.groups, run)
+        p = b.groups[0].problem(coef, grad_eps=ge, grad_sig=gs)
+        p.S = b.S
+        p.gshot = b.gshot.data_ptr()
+        check(lib.fgpr_reduce_shots(ctypes.byref(p), _stream(dev)), "reduce shots")
+        return finish_reference(G, ge, gs, want_er, want_se)
     if b.tb:
         b.gshot_buffer()
     p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 8
+#define FGPR_ABI_VERSION 9
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -81,7 +81,11 @@ extern "C" {
  *             slot(n) = n - snap_base, or n % snap_mod when snap_mod > 0
  *             (forward output, adjoint input); NULL = do not store.
  *   resid     S*nt*nrx floats: dL/dtraces (adjoint input, back.cu:30).
- *   grad_eps, grad_sig   NI*NJ doubles, accumulated (+=) by fgpr_adjoint.
+ *   grad_eps, grad_sig   NI*NJ doubles, accumulated (+=) by fgpr_adjoint.  The
+ *                        time-blocked reference adjoint accepts both null: it
+ *                        then leaves its shots' sums in gshot for a later
+ *                        fgpr_reduce_shots (shot groups of one batch on
+ *                        separate streams, reduced once in shot order).
  *   ctr       one device int32 used as the step counter of captured graphs.
  */
 typedef struct fgpr_problem {
@@ -113,7 +117,8 @@ typedef struct fgpr_problem {
     const int32_t* tb_rx_idx;   /*   entries s*nrx + r */
     const int32_t* tb_inj_ptr;  /* adjoint: receivers inside each (shot, tile) region */
     const int32_t* tb_inj_idx;
-    double* gshot;              /* adjoint: 2*S*NI*NJ per-shot FP64 partial gradients (zeroed by caller) */
+    double* gshot;              /* adjoint: S x 2 x NI x NJ per-shot FP64 partial gradients, shot-major
+                                   (a shot range is a contiguous slice); zeroed by the caller */
     float* scratch;             /* exact adjoint: 7*S*(NI+1)*pitch floats, zeroed by the caller */
     int32_t snap_base;          /* step of snapshot slot 0 (checkpointed segments; 0 otherwise) */
     int32_t nsnap;              /* snapshot slots in `snaps` (nt when it holds every step) */

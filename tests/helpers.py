@@ -46,7 +46,7 @@ def product_tensors(wl: Workload, device, er=None, se=None):
 
 def run_product(wl: Workload, device, exec_mode="graph", resid=None, want=("er", "se"),
                 graph_chunk=32, tv=False, model=None, time_block=None, gradient="reference",
-                checkpoint=None):
+                checkpoint=None, shot_streams=2):
     """Forward (and, given ``resid``, backward) through fastgpr.compute.
 
     Returns traces (numpy) and a dict of normalised gradients."""
@@ -63,7 +63,7 @@ def run_product(wl: Workload, device, exec_mode="graph", resid=None, want=("er",
                                  customreceiver=torch.tensor(wl.rx),
                                  exec_mode=exec_mode, graph_chunk=graph_chunk,
                                  time_block=time_block, gradient=gradient,
-                                 checkpoint=checkpoint)
+                                 checkpoint=checkpoint, shot_streams=shot_streams)
     grads = {}
     if resid is not None:
         traces.backward(torch.tensor(np.asarray(resid, np.float32)).to(device))

@@ -110,6 +110,20 @@ def test_time_blocked_matches_per_step_kernels(hip_device, tb):
         assert rel_l2(gb[k], ga[k]) < 1e-12
 
 
+@pytest.mark.parametrize("groups", [2, 3, 5])
+def test_shot_groups_on_streams_are_bitwise_one_batch(hip_device, groups):
+    """Shot groups swept on separate streams give the traces and the
+    (shot-order reduced) gradients of one batch, bitwise."""
+    from fastgpr.workloads import small_case
+    wl = small_case(NI=70, NJ=90, nt=157, S=5)
+    resid = _resid(wl)
+    print("groups", groups, "shots", wl.S, flush=True)
+    a, ga = run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6), shot_streams=1)
+    b, gb = run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6), shot_streams=groups)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ga["er"], gb["er"]) and np.array_equal(ga["se"], gb["se"])
+
+
 def test_time_blocked_eager_equals_graph(hip_device):
     from fastgpr.workloads import small_case
     wl = small_case(NI=50, NJ=130, nt=90)
