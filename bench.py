@@ -281,8 +281,9 @@ def profile_kernels(wl, er, se, mr, src, rx, tb, reps, groups=1):
                 "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": best, "avg_launch_us": round(d["us"], 2),
                 "bytes_per_launch": int(d["bytes"])}
-    if tb and groups > 1 and wl.S > 1:
-        sg = S_.ShotGroups(G, True, tb, min(groups, wl.S))
+    groups = min(groups, wl.S, S_.max_shot_groups())
+    if tb and groups > 1:
+        sg = S_.ShotGroups(G, True, tb, groups)
         sg.gshot_buffer()
         probs = []
         for g in sg.groups:

@@ -27,7 +27,7 @@ _cache: dict = {}
 
 def _key(G) -> str:
     name = torch.cuda.get_device_name(G.device)
-    groups = max(1, min(int(G.options.get("shot_streams", 3) or 1), G.S))
+    groups = max(1, min(int(G.options.get("shot_streams", 3) or 1), G.S, _solver.max_shot_groups()))
     return f"{name}|{G.NI}x{G.NJ}|P{G.pmlthick}|S{G.S}|rx{G.nrx}|g{groups}"
 
 

@@ -8,6 +8,7 @@ allocator and any HBM budget see it) and handed to libfastgpr by pointer.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -278,13 +279,27 @@ def _fan_out(device, groups, run):
         main.wait_event(done)
 
 
+def max_shot_groups() -> int:
+    """Side streams that get a hardware queue of their own: HIP gives a
+    process GPU_MAX_HW_QUEUES queues (4 by default) and the calling stream
+    keeps one.  Streams beyond that share queues, and a shared queue runs
+    its kernels in order: four groups on four side streams measured 93
+    Gcell-updates/s on C2 against 150 for three."""
+    try:
+        q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        q = 4
+    return max(1, q - 1)
+
+
 def shot_groups(G) -> int:
     """Shot groups of a full-history time-blocked sweep (option
-    ``shot_streams``, default 3; 1 = one batch on the current stream)."""
+    ``shot_streams``, default 3; 1 = one batch on the current stream),
+    at most one per spare hardware queue and one per shot."""
     n = int(G.options.get("shot_streams", 3) or 1)
     if not G.time_block or G.options.get("gradient", "reference") == "exact":
         return 1
-    return max(1, min(n, G.S))
+    return max(1, min(n, G.S, max_shot_groups()))
 
 
 def source_scale(G) -> torch.Tensor:

@@ -98,3 +98,25 @@ def test_compute_refuses_cpu_device():
                         time_windows=1e-8, pmlthick=5,
                         customsource=torch.tensor([[[15, 15, 0]]]),
                         customreceiver=torch.tensor([[[10, 10, 0]]]))
+
+
+def test_shot_groups_are_held_to_spare_hardware_queues(monkeypatch):
+    """Shot groups of a time-blocked sweep: the requested count, at most one
+    per shot and one per hardware queue beside the calling stream's; one
+    batch for the per-step kernels and the exact gradient."""
+    from types import SimpleNamespace
+    from fastgpr import solver
+    G = lambda S=13, tb=(64, 1024, 6), **opt: SimpleNamespace(S=S, time_block=tb, options=opt)
+    monkeypatch.delenv("GPU_MAX_HW_QUEUES", raising=False)
+    assert solver.max_shot_groups() == 3
+    assert solver.shot_groups(G()) == 3                       # default
+    assert solver.shot_groups(G(shot_streams=2)) == 2
+    assert solver.shot_groups(G(shot_streams=5)) == 3
+    assert solver.shot_groups(G(S=2, shot_streams=3)) == 2
+    assert solver.shot_groups(G(shot_streams=1)) == 1
+    assert solver.shot_groups(G(tb=None, shot_streams=3)) == 1
+    assert solver.shot_groups(G(gradient="exact", shot_streams=3)) == 1
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "2")
+    assert solver.shot_groups(G(shot_streams=3)) == 1
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8")
+    assert solver.shot_groups(G(shot_streams=5)) == 5
