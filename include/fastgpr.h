@@ -225,8 +225,10 @@ int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* strea
  *   fgpr_tb_launch_table launches it from a device copy of that table;
  *   fgpr_tb_settle       copies a problem's state back from the partner
  *                       buffers after an odd number of its blocks;
- *   fgpr_reduce_shots    grad_eps/grad_sig += sum over shots of gshot.
- * Supported variants: (32,512,{2,4,6}), (64,1024,{2,4,6}), (64,512,{4,6}).
+ *   fgpr_reduce_shots    grad_eps/grad_sig += sum over shots of gshot, in shot
+ *                       order (also the deferred reduction of shot groups,
+ *                       see grad_eps above).
+ * Supported variants: (32,512,{2,4,6}), (64,1024,{2,4,6}), (64,512,{2,4,6,8}).
  */
 typedef struct fgpr_tb_item {
     const fgpr_problem* prob;   /* host pointer to the problem descriptor */
