@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--exec", dest="exec_mode", default="graph", choices=["graph", "eager"])
-    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=None,
+                    help="steps per captured graph (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-block", default="auto",
                     help="'auto' (autotuned), 'none' (per-step kernels) or rows,threads,steps")
@@ -197,7 +198,8 @@ def main():
         "config": {"workload": wl.name, "grid": [wl.NI, wl.NJ], "dx_m": wl.dx, "cpml": wl.P,
                    "nt": wl.nt, "shots": wl.S, "receivers": int(wl.rx.shape[1]),
                    "cell_updates_per_gradient": cu, "exec": args.exec_mode,
-                   "graph_chunk": args.chunk,
+                   "graph_chunk": (args.chunk if args.chunk else
+                                   fastgpr.solver.TB_GRAPH_LAUNCHES * tb_used[2] if tb_used else 32),
                    "time_block": list(tb_used) if tb_used else None},
         "roofline": roofline,
         "cpu_baseline": cpu,

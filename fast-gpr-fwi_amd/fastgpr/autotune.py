@@ -1,12 +1,14 @@
 """Autotuner for the time-blocked kernels.
 
 For one grid (NI, NJ, P) and batch size S it times every compiled variant
-(region rows x threads x steps per launch) on a short forward + adjoint run
-on scratch buffers and keeps the fastest.  Results are cached per process
-and, when FASTGPR_TUNE_CACHE names a file, in that JSON file.
+(region rows x threads x steps per launch): repeated forward and adjoint
+launches of one block on scratch buffers, split into the shot groups the
+sweep will use, and keeps the fastest per step.  Results are cached per
+process and, when FASTGPR_TUNE_CACHE names a file, in that JSON file.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import json
 import os
@@ -15,6 +17,7 @@ import torch
 
 from . import _native
 from . import solver as _solver
+from ._native import lib
 
 CANDIDATES = ([(32, 256, k) for k in (1, 2, 3, 4, 6)]
               + [(64, 512, k) for k in (1, 2, 3, 4, 6, 8)]
@@ -46,25 +49,59 @@ def _save_file():
             json.dump({k: list(v) for k, v in _cache.items()}, f, indent=1)
 
 
-def time_variant(G, er, se, mr, tb, steps: int, reps: int = 2) -> float:
-    """Seconds per time step (forward + adjoint) of one variant."""
-    g = dataclasses.replace(G, iterations=steps, srcwaves=G.srcwaves[:, :steps].contiguous(),
-                            time_block=tb, exec_mode="eager")
-    best = float("inf")
-    for _ in range(reps + 1):
+def time_variant(G, er, se, mr, tb, reps: int = 12) -> float:
+    """Seconds per time step (forward + adjoint launch of one block, divided
+    by its steps) of one variant, as the sweep runs it: the batch split into
+    the same shot groups on their streams, each launching its block ``reps``
+    times back to back (fgpr_profile_kernel) after a warm-up; HIP events on
+    the calling stream bracket the fan-out and the join.  Buffers are
+    allocated before the timed region."""
+    K = int(tb[2])
+    nt = 4 * K
+    g = dataclasses.replace(G, iterations=nt, srcwaves=G.srcwaves[:, :nt].contiguous(),
+                            time_block=tb)
+    dev = g.device
+    se_f = torch.clamp(se.detach(), min=0, max=1e6)
+    coef = _solver.cell_coeffs(er, se_f, mr, g.dt32, g.dx32, False, dev)
+    n = _solver.shot_groups(g)
+    if n > 1:
+        sg = _solver.ShotGroups(g, True, tb, n)
+        sg.gshot_buffer()
+        groups = sg.groups
+    else:
+        groups = [_solver.Batch(g, store=True, tb=tb)]
+        groups[0].gshot_buffer()
+    resid = torch.zeros((g.S, nt, int(g.receiver.shape[1])), dtype=torch.float32, device=dev)
+    ge = torch.zeros((g.NI, g.NJ), dtype=torch.float64, device=dev)
+    gs = torch.zeros_like(ge)
+    probs = [b.problem(coef, srcscale=_solver.emitting_sources(g, b), resid=resid[b.lo:b.hi],
+                       grad_eps=ge, grad_sig=gs) for b in groups]
+    for b in groups:
+        b.snaps.zero_()
+    main = torch.cuda.current_stream(dev)
+    total = 0.0
+    for kid, step in ((_native.KERNEL_TB_FWD, K), (_native.KERNEL_TB_ADJ, 2 * K)):
+        def run(b, st, r):
+            _native.check(lib.fgpr_profile_kernel(ctypes.byref(probs[groups.index(b)]), kid, step,
+                                                  r, st), "profile")
+        if n > 1:
+            _solver._fan_out(dev, groups, lambda b, st: run(b, st, 2))
+        else:
+            run(groups[0], ctypes.c_void_p(main.cuda_stream), 2)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        _, b = _solver.forward(g, er, se, mr, store=True)
-        resid = torch.zeros((b.S, b.nt, b.nrx), dtype=torch.float32, device=g.device)
-        _solver.adjoint(g, b, er, se, mr, resid)
-        e1.record()
+        e0.record(main)
+        if n > 1:
+            _solver._fan_out(dev, groups, lambda b, st: run(b, st, reps))
+        else:
+            run(groups[0], ctypes.c_void_p(main.cuda_stream), reps)
+        e1.record(main)
         e1.synchronize()
-        best = min(best, e0.elapsed_time(e1) / 1e3)
-    return best / (2 * steps)
+        total += e0.elapsed_time(e1) / 1e3 / reps
+    return total / K
 
 
-def choose(G, er, se, mr, candidates=None, steps: int = 48) -> tuple:
+def choose(G, er, se, mr, candidates=None) -> tuple:
     """Fastest time-blocked variant for this grid and batch (cached)."""
     _load_file()
     key = _key(G)
@@ -74,7 +111,7 @@ def choose(G, er, se, mr, candidates=None, steps: int = 48) -> tuple:
     for tb in candidates or CANDIDATES:
         try:
             _native.tb_tiles(G.NI, G.NJ, *tb)
-            results[tb] = time_variant(G, er, se, mr, tb, steps)
+            results[tb] = time_variant(G, er, se, mr, tb)
         except (RuntimeError, ValueError):
             continue
     best = min(results, key=results.get)

@@ -10,7 +10,9 @@ Extra keyword arguments (all default to the reference behaviour):
 
 * ``exec_mode``   -- "graph" (time loop replayed from captured HIP graphs) or
                      "eager" (one kernel sequence per step from the host loop).
-* ``graph_chunk`` -- steps per captured graph.
+* ``graph_chunk`` -- steps per captured graph (default: 32 for the per-step
+                     kernels and checkpointed segments, 80 launches for a
+                     full-history time-blocked sweep).
 * ``time_block``  -- None (one H and one E kernel per step), "auto" (the
                      autotuner picks a time-blocked variant for this grid) or
                      an explicit (region rows, threads, steps per launch).
@@ -68,7 +70,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             receiver_location=None, receiver_step=None, pmlthick=0, freq=None,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
-            exec_mode="graph", graph_chunk=32, time_block=None, gradient="reference",
+            exec_mode="graph", graph_chunk=None, time_block=None, gradient="reference",
             checkpoint=None, shot_streams=3):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
@@ -96,7 +98,7 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                    receiver_location, receiver_step, pmlthick, source_apmlitudes, step, er, se,
                    mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
                    nseg, customsource=None, customreceiver=None, exec_mode="graph",
-                   graph_chunk=32, time_block=None, gradient="reference",
+                   graph_chunk=None, time_block=None, gradient="reference",
                    checkpoint=None, shot_streams=3) -> FDTDGrid:
     """Build the solve description (compute.py:726-872).
 
@@ -141,10 +143,11 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
     return FDTDGrid(device=device, NI=NI, NJ=NJ, S=S, nsrc=int(src.shape[1]), nrx=int(rx.shape[1]),
                     dx=dx, dt=Gdt, iterations=nt, pmlthick=P, source=src, receiver=rx,
                     srcwaves=srcwaves, pml=pml, tv=tv, exec_mode=exec_mode,
-                    graph_chunk=int(graph_chunk),
+                    graph_chunk=32 if graph_chunk is None else int(graph_chunk),
                     time_block=time_block if time_block in (None, "auto") else tuple(time_block),
                     options={"gradient": gradient, "checkpoint": checkpoint,
-                             "shot_streams": int(shot_streams)})
+                             "shot_streams": int(shot_streams),
+                             "graph_chunk_set": graph_chunk is not None})
 
 
 class GPRFWI(torch.autograd.Function):

@@ -302,6 +302,19 @@ def shot_groups(G) -> int:
     return max(1, min(n, G.S, max_shot_groups()))
 
 
+TB_GRAPH_LAUNCHES = 80
+
+
+def sweep_chunk(G) -> int:
+    """Steps per captured graph of a full-history sweep: ``graph_chunk`` when
+    the caller set it or the per-step kernels run, else 80 time-blocked
+    launches per graph (measured on C2: 16 launches 148, 32 150, 80 151-152,
+    160-500 145-150 Gcell-updates/s)."""
+    if G.options.get("graph_chunk_set") or not G.time_block or G.time_block == "auto":
+        return G.graph_chunk
+    return TB_GRAPH_LAUNCHES * int(G.time_block[2])
+
+
 def source_scale(G) -> torch.Tensor:
     """w[n]*dl/(dx*dy*dz) per source row in float32, the expression of
     sourcereceiver.cu:73-75 with dl = dy = dz = dx (compute.py:571, 693)."""
@@ -336,14 +349,14 @@ def forward(G, er, se, mr, store: bool = True):
 
         def run(g, st):
             p = g.problem(coef, srcscale=emitting_sources(G, g))
-            check(lib.fgpr_forward(ctypes.byref(p), 0, g.nt, _MODES[G.exec_mode], G.graph_chunk, st),
+            check(lib.fgpr_forward(ctypes.byref(p), 0, g.nt, _MODES[G.exec_mode], sweep_chunk(G), st),
                   "forward")
         _fan_out(dev, b.groups, run)
         return b.traces, b
     b = Batch(G, store=store, tb=G.time_block)
     scale = emitting_sources(G, b)
     p = b.problem(coef, srcscale=scale)
-    check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], G.graph_chunk,
+    check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], sweep_chunk(G),
                            _stream(dev)), "forward")
     return b.traces, b
 
@@ -412,7 +425,7 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
             # the shot reduction is deferred (null gradients) to one pass below
             p = g.problem(coef, srcscale=getattr(g, "srcscale", None), resid=resid[g.lo:g.hi])
             check(lib.fgpr_adjoint(ctypes.byref(p), g.nt - 1, 1, _native.GRAD_REFERENCE,
-                                   _MODES[G.exec_mode], G.graph_chunk, st), "adjoint")
+                                   _MODES[G.exec_mode], sweep_chunk(G), st), "adjoint")
         if b.nt >= 2:
             _fan_out(dev, b.groups, run)
         p = b.groups[0].problem(coef, grad_eps=ge, grad_sig=gs)
@@ -425,7 +438,7 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
     if b.nt >= 2:
         check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 1, _native.GRAD_REFERENCE,
-                               _MODES[G.exec_mode], G.graph_chunk, _stream(dev)), "adjoint")
+                               _MODES[G.exec_mode], sweep_chunk(G), _stream(dev)), "adjoint")
     return finish_reference(G, ge, gs, want_er, want_se)
 
 
