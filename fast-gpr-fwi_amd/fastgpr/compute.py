@@ -33,6 +33,7 @@ Extra keyword arguments (all default to the reference behaviour):
 """
 from __future__ import annotations
 
+import atexit
 import time
 
 import torch
@@ -42,6 +43,47 @@ from . import checkpoint as _ckpt
 from . import grid as _grid
 from . import solver as _solver
 from .grid import FDTDGrid
+
+
+# The reference prints a warning when traces, grad_output or a gradient hold a
+# NaN (compute.py:898-899, 905-906, 924-925, 931-932).  Testing on the host
+# at that point would stall the launch queue (the host waits for the GPU
+# before it queues the adjoint); the tests here are queued on the device,
+# their flags copied to pinned host memory behind an event, and the warnings
+# printed once the flags have arrived: at the end of backward when they
+# already have, else at the next call (or at exit).
+_nan_pending: list = []
+
+
+def _watch_nan(*pairs):
+    """Queue NaN tests of (tensor, message) pairs without a host sync."""
+    pairs = [(t, msg) for t, msg in pairs if t is not None]
+    if not pairs:
+        return
+    flags = torch.stack([torch.isnan(t).any() for t, _ in pairs])
+    host = torch.empty(flags.shape, dtype=torch.bool, pin_memory=True)
+    host.copy_(flags, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    _nan_pending.append((ev, host, [msg for _, msg in pairs]))
+
+
+def report_nan_warnings(block: bool = False):
+    """Print the warnings of every NaN test whose flags have arrived (all of
+    them, waiting, when ``block``)."""
+    keep = []
+    for ev, host, msgs in _nan_pending:
+        if block or ev.query():
+            ev.synchronize()
+            for flag, msg in zip(host.tolist(), msgs):
+                if flag:
+                    print(msg)
+        else:
+            keep.append((ev, host, msgs))
+    _nan_pending[:] = keep
+
+
+atexit.register(lambda: report_nan_warnings(block=True) if torch.cuda.is_initialized() else None)
 
 
 class GPRFWIModel(torch.nn.Module):
@@ -79,6 +121,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
     er / se / mr are (nx, ny, 1) tensors, locations and steps are metres.
     se is clamped at 0 and er at 1 (differentiably) before the solve.
     """
+    report_nan_warnings()
     se = torch.clamp(se, min=0)
     er = torch.clamp(er, min=1)
     G = initialization(device, dx, ny, nx, nz, time_windows, source, source_location,
@@ -167,14 +210,12 @@ class GPRFWI(torch.autograd.Function):
         else:
             traces, batch = _solver.forward(G, er, se, mr, store=need_grad)
             ctx.batch = batch if need_grad else None
-        if torch.isnan(traces).any():
-            print("⚠️ rxs_gpu contains NaN!")
+        _watch_nan((traces, "⚠️ rxs_gpu contains NaN!"))
         return traces
 
     @staticmethod
     def backward(ctx, grad_output):
-        if torch.isnan(grad_output).any():
-            print("⚠️ grad_output contains NaN!")
+        _watch_nan((grad_output, "⚠️ grad_output contains NaN!"))
         er, se, mr = ctx.saved_tensors
         G = ctx.G
         exact = G.options.get("gradient") == "exact"
@@ -194,12 +235,11 @@ class GPRFWI(torch.autograd.Function):
             if er.requires_grad:
                 uer = total_variation(er.clone())
                 g_er += 0.01 * (2 * (er - uer))
-                if torch.isnan(g_er).any():
-                    print("⚠️ epsilon.grad contains NaN!")
+                _watch_nan((g_er, "⚠️ epsilon.grad contains NaN!"))
             if se.requires_grad:
                 use = total_variation(se.clone())
                 g_se += 0.01 * (2 * (se - use))
-                if torch.isnan(g_se).any():
-                    print("⚠️ sigma.grad contains NaN!")
+                _watch_nan((g_se, "⚠️ sigma.grad contains NaN!"))
             print(f"Total variation regularization took {time.time() - t0:.4f} seconds.")
+        report_nan_warnings()
         return None, g_er, g_se, None

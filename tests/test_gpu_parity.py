@@ -204,3 +204,24 @@ def test_exact_gradient_central_differences_16_cells(hip_device):
         e = rel_l2(a, f)
         print(f"FD check {k}: rel-L2(adjoint, central differences) over 16 cells = {e:.3e}")
         assert e <= 2e-2
+
+
+def test_nan_warnings_without_host_sync(hip_device, capsys):
+    """The reference's NaN warnings (compute.py:898-906) are still printed:
+    queued on the device, reported once their flags have arrived."""
+    import torch
+    from fastgpr.compute import report_nan_warnings
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=60)
+    er0 = wl.er0.copy()
+    er0[wl.NI // 2, wl.NJ // 2] = np.nan
+    resid = _resid(wl)
+    tr, _ = run_product(wl, hip_device, resid=resid, model=(er0, wl.se0), time_block=(64, 1024, 6))
+    report_nan_warnings(block=True)
+    out = capsys.readouterr().out
+    assert np.isnan(tr).any()
+    assert "rxs_gpu contains NaN" in out
+    # a clean run prints nothing
+    run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6))
+    report_nan_warnings(block=True)
+    assert "NaN" not in capsys.readouterr().out
