@@ -1220,6 +1220,20 @@ __global__ void __launch_bounds__(256) k_snap_pack(Dev d, int RI, int NT, int K,
         in ? Ez[((int64_t)sh * (d.NI + 1) + gi) * d.pitch + gj] : 0.f;
 }
 
+// Sequential FP32 sums of n strided elements of a and b (the CPML sigma_max
+// means, compute.py:278-281), one thread, in index order.
+__global__ void k_row_sums(const float* __restrict__ a, const float* __restrict__ b, int n, int stride,
+                           float* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    float sa = 0.f, sb = 0.f;
+    for (int j = 0; j < n; ++j) {
+        sa = sa + a[(int64_t)j * stride];
+        sb = sb + b[(int64_t)j * stride];
+    }
+    out[0] = sa;
+    out[1] = sb;
+}
+
 // grad[c] += sum_s gshot[s][p][c] in shot order (deterministic).
 __global__ void k_reduce_shots(const double* __restrict__ gshot, int S, int64_t ncell,
                                double* grad_eps, double* grad_sig) {
@@ -1963,6 +1977,13 @@ int fgpr_reduce_shots(const fgpr_problem* p, void* stream) {
     const int64_t ncell = (int64_t)p->NI * p->NJ;
     hipLaunchKernelGGL(k_reduce_shots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, p->gshot, p->S, ncell, p->grad_eps, p->grad_sig);
+    HIP_TRY(hipGetLastError());
+    return FGPR_OK;
+}
+
+int fgpr_row_sums(const float* a, const float* b, int n, int stride, float* out, void* stream) {
+    if (!a || !b || !out || n < 1 || stride < 1) return fail(FGPR_EINVAL, "bad row-sum arguments");
+    hipLaunchKernelGGL(k_row_sums, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, n, stride, out);
     HIP_TRY(hipGetLastError());
     return FGPR_OK;
 }

@@ -94,6 +94,8 @@ SIGNATURES = {
     "fgpr_reduce_shots": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
     "fgpr_tv_workspace_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
+    "fgpr_row_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
     "fgpr_snap_slot_floats": (ctypes.c_int64, [ctypes.POINTER(Problem)]),
     "fgpr_snap_pack": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p]),

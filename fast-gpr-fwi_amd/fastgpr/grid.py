@@ -90,25 +90,45 @@ def pml_profile(P: int, dx, dt, er, mr) -> torch.Tensor:
     coefficients (compute.py:211-222).  All four slabs share one CFS object,
     so one set of vectors serves x0, y0, xmax and ymax (compute.py:178, 829).
     """
-    er0 = _cpu(er)[0, :, 0]
-    mr0 = _cpu(mr)[0, :, 0]
-    sumer = 0
-    summr = 0
-    for j in range(er0.shape[0]):       # sequential sum, as compute.py:278-281
-        sumer = sumer + er0[j]
-        summr = summr + mr0[j]
-    n = er0.shape[0]
+    n = er.shape[1]
+    if er.is_cuda:
+        # on the device (no copy of the model to the host, no stall of the
+        # launch queue); the sums are sequential FP32 like the loop below
+        from . import _native
+        er, mr = er.detach().float(), mr.detach().float().to(er.device)
+        if mr.stride(1) != er.stride(1):
+            mr = mr.contiguous()
+            er = er.contiguous()
+        sums = torch.empty(2, dtype=torch.float32, device=er.device)
+        _native.check(_native.lib.fgpr_row_sums(
+            er[0, :, 0].data_ptr(), mr[0, :, 0].data_ptr(), n, er.stride(1), sums.data_ptr(),
+            torch.cuda.current_stream(er.device).cuda_stream), "row sums")
+        sumer, summr = sums[0], sums[1]
+        dev = er.device
+    else:
+        er0 = _cpu(er)[0, :, 0]
+        mr0 = _cpu(mr)[0, :, 0]
+        sumer = 0
+        summr = 0
+        for j in range(er0.shape[0]):       # sequential sum, as compute.py:278-281
+            sumer = sumer + er0[j]
+            summr = summr + mr0[j]
+        dev = torch.device("cpu")
     er_mean = sumer / n
     mr_mean = summr / n
     d = _cpu(dx)
+    if isinstance(d, torch.Tensor):
+        d = d.to(dev)
     smax = (0.8 * (4 + 1)) / (Z0 * d * torch.sqrt(er_mean * mr_mean))
     L = P + 1
-    tmp = (torch.linspace(0, (L - 1) + 0.5, steps=2 * L) / (L - 1)) ** 4
+    tmp = (torch.linspace(0, (L - 1) + 0.5, steps=2 * L, device=dev) / (L - 1)) ** 4
     Esig = (tmp[0:-1:2] * (smax - 0) + 0)[:-1]
     Hsig = (tmp[1::2] * (smax - 0) + 0)[:-1]
-    kap = torch.ones(P)
-    alp = torch.zeros(P)
+    kap = torch.ones(P, device=dev)
+    alp = torch.zeros(P, device=dev)
     dtc = _cpu(dt)
+    if isinstance(dtc, torch.Tensor):
+        dtc = dtc.to(dev)
     rows = []
     for sig in (Esig, Hsig):
         tmpd = (2 * EPS0 * kap) + dtc * (alp * kap + sig)

@@ -27,6 +27,15 @@ def _stream(device) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def to_device(t: torch.Tensor, device) -> torch.Tensor:
+    """Host tensor -> device through pinned memory, asynchronously: a copy
+    from pageable memory first waits for all queued device work, which would
+    idle the GPU while the host prepares the next sweep."""
+    if t.device.type != "cpu":
+        return t.to(device)
+    return t.contiguous().pin_memory().to(device, non_blocking=True)
+
+
 def require_hip(device) -> torch.device:
     device = torch.device(device)
     if device.type != "cuda":
@@ -116,13 +125,14 @@ class Batch:
         self.fields = torch.zeros((3, self.S, self.NI + 1, self.pitch), dtype=torch.float32, device=dev)
         npsi = max(_native.psi_floats(self.NI, self.NJ, self.P, self.S), 1)
         self.psi = torch.zeros(npsi, dtype=torch.float32, device=dev)
-        self.pml = (G.pml.to(dev).contiguous() if self.P > 0
+        self.pml = (to_device(G.pml, dev).contiguous() if self.P > 0
                     else torch.zeros(8, dtype=torch.float32, device=dev))
         from .grid import device_cells
         if G.source.shape[0] < self.hi or G.receiver.shape[0] < self.hi:
             raise ValueError(f"source/receiver arrays have fewer rows than the {self.hi} shots")
-        self.src = device_cells(G.source[self.lo:self.hi], self.NI, self.NJ, "source").to(dev)
-        self.rx = device_cells(G.receiver[self.lo:self.hi], self.NI, self.NJ, "receiver").to(dev)
+        self.src = to_device(device_cells(G.source[self.lo:self.hi], self.NI, self.NJ, "source"), dev)
+        rx_host = device_cells(G.receiver[self.lo:self.hi], self.NI, self.NJ, "receiver")
+        self.rx = to_device(rx_host, dev)
         self.nsrc = self.src.shape[1]
         self.nrx = self.rx.shape[1]
         self.ctr = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -138,9 +148,9 @@ class Batch:
         if self.tb:
             self.fields2 = torch.zeros_like(self.fields)
             self.psi2 = torch.zeros_like(self.psi)
-            t = tile_tables(self.rx.cpu().numpy(), self.NI, self.NJ, self.tb)
+            t = tile_tables(rx_host.numpy(), self.NI, self.NJ, self.tb)
             self.tiles = (t["ti"], t["tj"])
-            as_dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+            as_dev = lambda a: to_device(torch.from_numpy(np.ascontiguousarray(a)), dev)
             self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
                               as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
         self.snaps = (torch.empty((self.nt,) + self.snap_slot_shape(), dtype=torch.float32, device=dev)
@@ -330,10 +340,11 @@ def emitting_sources(G, b: Batch) -> torch.Tensor:
     scale = source_scale(G)
     emit = (scale != 0).any(dim=1)
     if not bool(emit.all()):
-        b.src = b.src[:, emit.to(b.src.device)].contiguous()
+        keep = to_device(emit.nonzero().flatten(), b.src.device)
+        b.src = b.src.index_select(1, keep).contiguous()
         b.nsrc = int(b.src.shape[1])
         scale = scale[emit]
-    scale = scale.contiguous().to(G.device)
+    scale = to_device(scale, G.device)
     b.srcscale = scale
     return scale
 

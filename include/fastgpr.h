@@ -257,6 +257,12 @@ int64_t fgpr_tv_workspace_bytes(int M, int N);
 int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
             void* workspace, int64_t bytes, float* out, void* stream);
 
+/* out[0] = a[0] + a[s] + a[2s] + ..., out[1] the same over b (n terms,
+ * stride s, FP32, in index order): the sequential sums of the first model
+ * row that set the CPML sigma_max (compute.py:278-281), on the device so
+ * that a call needs no device-to-host copy of the model. */
+int fgpr_row_sums(const float* a, const float* b, int n, int stride, float* out, void* stream);
+
 /* Floats per snapshot slot in the problem's layout (row-major S*NI*pitch, or
  * S*tiles*tb_ri*64 when snap_tiled). */
 int64_t fgpr_snap_slot_floats(const fgpr_problem* p);

@@ -225,3 +225,19 @@ def test_nan_warnings_without_host_sync(hip_device, capsys):
     run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6))
     report_nan_warnings(block=True)
     assert "NaN" not in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("P,NJ", [(10, 24), (20, 200), (3, 7)])
+def test_pml_profile_on_device_matches_host(hip_device, P, NJ):
+    """The CPML profile computed from device-resident er / mr (no copy to the
+    host) equals the host computation (sequential sums, same expressions) to
+    an ulp or two (the device's pow / division round differently)."""
+    import torch
+    from fastgpr import grid as g
+    rng = np.random.default_rng(P + NJ)
+    er = torch.tensor(1.0 + 9.0 * rng.random((2 * P + 5, NJ, 1)), dtype=torch.float32)
+    mr = torch.tensor(1.0 + 0.5 * rng.random((2 * P + 5, NJ, 1)), dtype=torch.float32)
+    host = g.pml_profile(P, 0.025, 4.1e-11, er, mr)
+    dev = g.pml_profile(P, 0.025, 4.1e-11, er.to(hip_device), mr.to(hip_device))
+    assert dev.device.type == "cuda" and dev.shape == host.shape
+    np.testing.assert_allclose(dev.cpu().numpy(), host.numpy(), rtol=1.2e-6, atol=0)
