@@ -150,11 +150,13 @@ def main():
     for _ in range(args.warmup):
         gradient_step()
     torch.cuda.synchronize()
+    fastgpr.solver.record_sweeps(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         gradient_step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    sweeps = fastgpr.solver.record_sweeps(False)
     assert torch.isfinite(er.grad).all() and torch.isfinite(se.grad).all()
 
     cu = wl.cell_updates_per_gradient()
@@ -176,6 +178,23 @@ def main():
     roofline, per_kernel = profile_kernels(wl, er.detach(), se.detach(), mr, src, rx, tb_used,
                                            args.profile_reps, args.shot_streams)
     roofline["traffic"] = measured_traffic(args.traffic, roofline["kernel"])
+    # Sweeps of the timed region as run (shot groups on their streams): HIP
+    # events on the calling stream before each fan-out and after its join.
+    if tb_used and sweeps:
+        blocks = -(-wl.nt // tb_used[2])
+        per = {}
+        for kind, a, b in sweeps:
+            per.setdefault(kind, []).append(a.elapsed_time(b) * 1e3 / blocks)
+        us = {k: sum(v) / len(v) for k, v in per.items()}
+        adj = us.get("adjoint")
+        if adj:
+            gbs = roofline["bytes_per_launch"] / adj / 1e3
+            roofline["effective"] = {
+                "what": "timed-region sweeps: wall time per whole-batch block, all shot groups",
+                "groups": roofline.get("effective", {}).get("groups", 1),
+                "us_per_block": round(adj, 2), "achieved": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "forward_us_per_block": round(us.get("forward", 0.0), 2)}
 
     # ---- CPU baseline: the numpy FP64 oracle on a bounded sample
     cpu = None

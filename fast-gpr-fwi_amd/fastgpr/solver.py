@@ -302,6 +302,38 @@ def max_shot_groups() -> int:
     return max(1, q - 1)
 
 
+_sweep_log = None
+
+
+def record_sweeps(on: bool):
+    """Start recording a pair of HIP events on the calling stream around
+    every full-history sweep (forward and reference adjoint: from before the
+    fan-out to after the join), or stop and return the records
+    [(kind, start, end), ...].  Measurement only."""
+    global _sweep_log
+    if on:
+        _sweep_log = []
+        return None
+    out, _sweep_log = _sweep_log, None
+    return out
+
+
+def _sweep_begin(device):
+    if _sweep_log is None:
+        return None
+    ev = torch.cuda.Event(enable_timing=True)
+    ev.record(torch.cuda.current_stream(device))
+    return ev
+
+
+def _sweep_end(device, kind, start):
+    if start is None or _sweep_log is None:
+        return
+    ev = torch.cuda.Event(enable_timing=True)
+    ev.record(torch.cuda.current_stream(device))
+    _sweep_log.append((kind, start, ev))
+
+
 def shot_groups(G) -> int:
     """Shot groups of a full-history time-blocked sweep (option
     ``shot_streams``, default 3; 1 = one batch on the current stream),
@@ -357,18 +389,22 @@ def forward(G, er, se, mr, store: bool = True):
     n = shot_groups(G)
     if n > 1:
         b = ShotGroups(G, store, G.time_block, n)
+        probs = [g.problem(coef, srcscale=emitting_sources(G, g)) for g in b.groups]
 
         def run(g, st):
-            p = g.problem(coef, srcscale=emitting_sources(G, g))
-            check(lib.fgpr_forward(ctypes.byref(p), 0, g.nt, _MODES[G.exec_mode], sweep_chunk(G), st),
-                  "forward")
+            check(lib.fgpr_forward(ctypes.byref(probs[b.groups.index(g)]), 0, g.nt,
+                                   _MODES[G.exec_mode], sweep_chunk(G), st), "forward")
+        t0 = _sweep_begin(dev)
         _fan_out(dev, b.groups, run)
+        _sweep_end(dev, "forward", t0)
         return b.traces, b
     b = Batch(G, store=store, tb=G.time_block)
     scale = emitting_sources(G, b)
     p = b.problem(coef, srcscale=scale)
+    t0 = _sweep_begin(dev)
     check(lib.fgpr_forward(ctypes.byref(p), 0, b.nt, _MODES[G.exec_mode], sweep_chunk(G),
                            _stream(dev)), "forward")
+    _sweep_end(dev, "forward", t0)
     return b.traces, b
 
 
@@ -432,13 +468,18 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     if isinstance(b, ShotGroups):
         b.gshot_buffer()
 
+        # the shot reduction is deferred (null gradients) to one pass below
+        probs = [g.problem(coef, srcscale=getattr(g, "srcscale", None), resid=resid[g.lo:g.hi])
+                 for g in b.groups]
+
         def run(g, st):
-            # the shot reduction is deferred (null gradients) to one pass below
-            p = g.problem(coef, srcscale=getattr(g, "srcscale", None), resid=resid[g.lo:g.hi])
-            check(lib.fgpr_adjoint(ctypes.byref(p), g.nt - 1, 1, _native.GRAD_REFERENCE,
-                                   _MODES[G.exec_mode], sweep_chunk(G), st), "adjoint")
+            check(lib.fgpr_adjoint(ctypes.byref(probs[b.groups.index(g)]), g.nt - 1, 1,
+                                   _native.GRAD_REFERENCE, _MODES[G.exec_mode], sweep_chunk(G), st),
+                  "adjoint")
         if b.nt >= 2:
+            t0 = _sweep_begin(dev)
             _fan_out(dev, b.groups, run)
+            _sweep_end(dev, "adjoint", t0)
         p = b.groups[0].problem(coef, grad_eps=ge, grad_sig=gs)
         p.S = b.S
         p.gshot = b.gshot.data_ptr()
@@ -448,8 +489,10 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
         b.gshot_buffer()
     p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
     if b.nt >= 2:
+        t0 = _sweep_begin(dev)
         check(lib.fgpr_adjoint(ctypes.byref(p), b.nt - 1, 1, _native.GRAD_REFERENCE,
                                _MODES[G.exec_mode], sweep_chunk(G), _stream(dev)), "adjoint")
+        _sweep_end(dev, "adjoint", t0)
     return finish_reference(G, ge, gs, want_er, want_se)
 
 
