@@ -176,7 +176,7 @@ def main():
     else:
         tb_used = tb
     roofline, per_kernel = profile_kernels(wl, er.detach(), se.detach(), mr, src, rx, tb_used,
-                                           args.profile_reps, args.shot_streams)
+                                           args.profile_reps)
     roofline["traffic"] = measured_traffic(args.traffic, roofline["kernel"])
     # Sweeps of the timed region as run (shot groups on their streams): HIP
     # events on the calling stream before each fan-out and after its join.
@@ -191,7 +191,7 @@ def main():
             gbs = roofline["bytes_per_launch"] / adj / 1e3
             roofline["effective"] = {
                 "what": "timed-region sweeps: wall time per whole-batch block, all shot groups",
-                "groups": roofline.get("effective", {}).get("groups", 1),
+                "groups": max(1, min(args.shot_streams, wl.S, fastgpr.solver.max_shot_groups())),
                 "us_per_block": round(adj, 2), "achieved": round(gbs, 1),
                 "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "forward_us_per_block": round(us.get("forward", 0.0), 2)}
@@ -223,8 +223,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["us"], 2) for k, v in per_kernel.items()},
-        "kernels_effective_us": {k: round(v["eff_us"], 2) for k, v in per_kernel.items()
-                                 if "eff_us" in v},
+
     }
     if args.plain:
         out["plain_path"] = plain_path(wl, er.detach(), se.detach(), mr, obs, src, rx, cu)
@@ -242,17 +241,12 @@ KERNEL_NAMES = {
 }
 
 
-def profile_kernels(wl, er, se, mr, src, rx, tb, reps, groups=1):
+def profile_kernels(wl, er, se, mr, src, rx, tb, reps):
     """Average launch duration of each kernel of the path (HIP events on the
     stream the kernels run on), its algorithmic bytes and GB/s; the dominant
-    kernel is the one with the largest share of a gradient's time.
-
-    The roofline is that of one launch over the whole batch on one stream.
-    With ``groups`` > 1 (the shot groups the timed run sweeps on separate
-    streams) each kernel also gets ``eff_us``: the time per whole-batch
-    block when every group launches its block concurrently on its own
-    stream (events on the launching stream before the fan-out and after the
-    join), reported as ``roofline["effective"]``."""
+    kernel is the one with the largest share of a gradient's time.  The
+    roofline is that of one launch over the whole batch on one stream (the
+    shot groups' concurrent sweeps are timed in the timed region itself)."""
     import ctypes
     import torch
     import fastgpr
@@ -302,31 +296,6 @@ def profile_kernels(wl, er, se, mr, src, rx, tb, reps, groups=1):
                 "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": best, "avg_launch_us": round(d["us"], 2),
                 "bytes_per_launch": int(d["bytes"])}
-    groups = min(groups, wl.S, S_.max_shot_groups())
-    if tb and groups > 1:
-        sg = S_.ShotGroups(G, True, tb, groups)
-        sg.gshot_buffer()
-        probs = []
-        for g in sg.groups:
-            sc = S_.emitting_sources(G, g)
-            probs.append(g.problem(coef, srcscale=sc, resid=resid[g.lo:g.hi], grad_eps=ge, grad_sig=gs))
-        for kid, (n, _) in kids.items():
-            def run(g, st, reps_=reps, kid_=kid, n_=n):
-                q = probs[sg.groups.index(g)]
-                _native.check(_native.lib.fgpr_profile_kernel(ctypes.byref(q), kid_, n_, reps_, st))
-            S_._fan_out(dev, sg.groups, lambda g, st: run(g, st, 4))
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            S_._fan_out(dev, sg.groups, run)
-            e1.record(stream)
-            e1.synchronize()
-            name = KERNEL_NAMES[kid].format(ri=tb[0], nt=tb[1], k=k)
-            per[name]["eff_us"] = e0.elapsed_time(e1) * 1e3 / reps
-        de = per[best]
-        gbs = de["bytes"] / de["eff_us"] / 1e3
-        roofline["effective"] = {"groups": len(sg.groups), "us_per_block": round(de["eff_us"], 2),
-                                 "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     return roofline, per
 
 
