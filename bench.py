@@ -28,7 +28,7 @@ METRIC = "gradient Gcell-updates/s (fwd+adjoint) + p50/p99 request latency; HBM 
 HBM_PEAK_GBS = 8000.0
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--traffic", default=str(Path(__file__).resolve().parent / "profiles" /
                                              "traffic_c2_crosshole.json"),
                     help="measured HBM bytes per launch (from the counter runs), if present")
-    return ap.parse_args()
+    return ap.parse_args(argv)
 
 
 def step_bytes(S: int, NI: int, NJ: int, P: int) -> float:

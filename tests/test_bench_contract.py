@@ -1,0 +1,34 @@
+"""bench.py's contract on the CPU: the one-GPU guard and the algorithmic
+bytes the roofline divides by (hand-computed for the cross-hole batch)."""
+import subprocess
+import sys
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(REPO))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+
+
+def test_algorithmic_bytes_of_a_time_blocked_launch():
+    import bench
+    # 40 B per cell-update (Ez, Hx, Hy read + write, three coefficients, the
+    # snapshot) and 16 B per CPML cell and direction; C2: 400 x 200 cells,
+    # 20-cell CPML on all sides, 13 shots, 6 steps per launch.
+    cells = 400 * 200
+    psi = 2 * 20 * 200 + 2 * 20 * 400
+    per_step = 13 * (40 * cells + 16 * psi)
+    assert bench.step_bytes(13, 400, 200, 20) == per_step
+    assert bench.kernel_bytes(5, 13, 400, 200, 20, 80, 6) == 6 * per_step == 279_552_000
+    assert bench.kernel_bytes(6, 13, 400, 200, 20, 80, 6) == 279_552_000
+
+
+def test_defaults():
+    import bench
+    a = bench.parse([])
+    assert (a.gpus, a.steps, a.warmup, a.shot_streams, a.time_block) == (1, 5, 1, 3, "auto")
