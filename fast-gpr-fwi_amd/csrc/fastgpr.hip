@@ -811,6 +811,24 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 #pragma unroll
     for (int c = 0; c < kWaveList / 64; ++c) rxent[c] = (c * 64 + lane < wn) ? sRx[w][c * 64 + lane] : 0;
 
+    // Adjoint: this wave's residual samples for every step of the launch,
+    // loaded with the set-up into one LDS row of kWaveList floats per step,
+    // so that the injection never waits on a global load inside the steps.
+    __shared__ float sRes[ADJ ? W * K * kWaveList : 1];
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < kWaveList / 64; ++c) {
+            if (c * 64 + lane < wn) {
+                const float* rp = d.resid + (size_t)s * d.nt * d.nrx + (rxent[c] & 0xFFFFF);
+                float rv[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) rv[k] = k < nsteps ? rp[(size_t)(n0 - k) * d.nrx] : 0.f;
+#pragma unroll
+                for (int k = 0; k < K; ++k) sRes[(w * K + k) * kWaveList + c * 64 + lane] = rv[k];
+            }
+        }
+    }
+
     // Cells of this thread holding a source (forward).
     unsigned srcbits = 0u;
     if (!ADJ) {
@@ -1065,7 +1083,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 for (int c = 0; c < kWaveList / 64; ++c) {
                     const int cnt = wn - c * 64 < 64 ? wn - c * 64 : 64;
                     const int mine = rxent[c];
-                    const float gv = lane < cnt ? d.resid[((size_t)s * d.nt + n) * d.nrx + (mine & 0xFFFFF)] : 0.f;
+                    const float gv = lane < cnt ? sRes[(w * K + k) * kWaveList + c * 64 + lane] : 0.f;
                     for (int q = 0; q < cnt; ++q) {
                         const int ent = __builtin_amdgcn_readlane(mine, q);
                         const float g = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gv), q));
