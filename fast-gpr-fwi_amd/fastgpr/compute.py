@@ -60,11 +60,17 @@ def _watch_nan(*pairs):
     pairs = [(t, msg) for t, msg in pairs if t is not None]
     if not pairs:
         return
-    flags = torch.stack([torch.isnan(t).any() for t, _ in pairs])
+    dev = pairs[0][0].device
+    if dev.type != "cuda":
+        for t, msg in pairs:
+            if torch.isnan(t).any():
+                print(msg)
+        return
+    flags = torch.stack([torch.isnan(t).any().to(dev) for t, _ in pairs])
     host = torch.empty(flags.shape, dtype=torch.bool, pin_memory=True)
     host.copy_(flags, non_blocking=True)
     ev = torch.cuda.Event()
-    ev.record()
+    ev.record(torch.cuda.current_stream(dev))
     _nan_pending.append((ev, host, [msg for _, msg in pairs]))
 
 
