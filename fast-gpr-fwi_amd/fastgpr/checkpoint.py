@@ -33,7 +33,9 @@ def interval(G, checkpoint) -> int | None:
 
     "auto" checkpoints only when the full history would take more than half
     of the free device memory, with C about sqrt(3 nt) rounded up to a whole
-    number of captured-graph chunks."""
+    number of captured-graph chunks.  A history under a quarter of the
+    device's memory is stored without asking the device for its free
+    memory."""
     if checkpoint is None or checkpoint is False:
         return None
     nt = G.iterations
@@ -43,6 +45,8 @@ def interval(G, checkpoint) -> int | None:
             _, _, ti, tj = _native.tb_tiles(G.NI, G.NJ, *G.time_block)
             slot = G.S * ti * tj * G.time_block[0] * 64          # time-blocked layout
         full = nt * slot * 4
+        if full <= 0.25 * torch.cuda.get_device_properties(G.device).total_memory:
+            return None
         free, _ = torch.cuda.mem_get_info(G.device)
         if full <= 0.5 * free:
             return None

@@ -22,8 +22,10 @@ Extra keyword arguments (all default to the reference behaviour):
 * ``checkpoint``  -- None (store every snapshot, as the reference), an
                      interval C (keep the solver state every C steps and
                      recompute snapshots segment by segment in the adjoint),
-                     or "auto" (checkpoint only when the history would not
-                     fit in half the free device memory).
+                     or "auto" (the default: store every snapshot, as the
+                     reference, unless the history would not fit in half the
+                     free device memory; checkpointed gradients are bitwise
+                     the stored ones).
 * ``shot_streams`` -- shot groups of a time-blocked sweep, each on its own
                      HIP stream so that the launches of one fill the CUs the
                      other's launch tail leaves idle (default 3; 1 = one
@@ -119,7 +121,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
             exec_mode="graph", graph_chunk=None, time_block=None, gradient="reference",
-            checkpoint=None, shot_streams=3):
+            checkpoint="auto", shot_streams=3):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
     Same arguments and meaning as compute.py:33-50: ``nx`` is the depth
@@ -148,7 +150,7 @@ def initialization(device, dx, ny, nx, nz, time_windows, source, source_location
                    mr, nsrc, nrx, dt, pml_formulation, freq, total_step, tv, Fixedreceiver,
                    nseg, customsource=None, customreceiver=None, exec_mode="graph",
                    graph_chunk=None, time_block=None, gradient="reference",
-                   checkpoint=None, shot_streams=3) -> FDTDGrid:
+                   checkpoint="auto", shot_streams=3) -> FDTDGrid:
     """Build the solve description (compute.py:726-872).
 
     ``nseg`` (the reference's unfinished segmenting, compute.py:543-553) and
