@@ -39,10 +39,16 @@ Pinning of each function (what checks it, in ``tests/test_oracle.py``):
 * ``forward``: lossless-box energy bound, mirror symmetry, source/receiver
   reciprocity, CPML reflection against a 4x larger domain, FP32-vs-FP64 drift,
   linearity in the source, and agreement with the independent C restatement.
-* ``adjoint_reference``: linearity in the residual (before normalisation),
-  unit norm, zero rows/columns the reference never reaches, and agreement
-  with the C restatement.  No closed form exists for the reference's
-  approximate gradient, so beyond these invariants it is "parity unpinned".
+* ``adjoint_reference`` (``tests/test_oracle_adjoint_pins.py``): in a
+  lossless medium without CPML the reference's backward loop is the exact
+  transpose of the forward with the adjoint fields scaled by uE1, so both of
+  its gradients equal dt/(eps0 dx) times ``adjoint_exact``'s epsilon gradient
+  for a residual divided by uE1 at the receivers and snapshots shifted one
+  step (their running sum for sigma); with loss and CPML its lambda fields
+  equal a forward run with the adjoint coefficients driven at the receivers
+  by the time-reversed residual times uE0/uE4.  Swapped phases or an
+  off-by-one residual index fail both.  Also linearity, unit norm, zero
+  rows/columns and agreement with the C restatement (``tests/test_oracle.py``).
 * ``adjoint_exact``: the dot-product test <J dm, r> = <dm, J^T r> and central
   finite differences of the forward map.
 """
@@ -351,7 +357,7 @@ def forward(c: Coeffs, pml: dict, P: int, dx: float, src: np.ndarray,
 
 def adjoint_reference(cadj: Coeffs, pml: dict, P: int, dx: float,
                       rx: np.ndarray, resid: np.ndarray, snaps: np.ndarray,
-                      dtype=np.float64):
+                      dtype=np.float64, hook=None):
     """Reference-mode adjoint (compute.py:1048-1172).
 
     lambda fields start at zero with the adjoint coefficients (uc.cu:60-91).
@@ -362,6 +368,7 @@ def adjoint_reference(cadj: Coeffs, pml: dict, P: int, dx: float,
     summed in float64 (the reference sums in float32; the build accumulates
     in float64).  Returns the unnormalised (g_eps, g_sig) with g_sig still
     missing its constant dt factor (it cancels under normalisation).
+    ``hook(n, state)``, if given, sees the lambda fields after step n.
     """
     f = dtype
     S, nt, nrx = resid.shape
@@ -380,6 +387,8 @@ def adjoint_reference(cadj: Coeffs, pml: dict, P: int, dx: float,
         lam = st.Ez[:, :NI, :NJ]
         ge += ((snaps[n] - snaps[n - 1]) * lam).astype(np.float64).sum(axis=0)
         gs += (snaps[n] * lam).astype(np.float64).sum(axis=0)
+        if hook is not None:
+            hook(n, st)
     return ge, gs
 
 
