@@ -1,5 +1,9 @@
-/* FP64 C restatement of the oracle's time stepping (forward and
- * reference-mode adjoint), parallel over shot rows with OpenMP.
+/* C restatement of the oracle's time stepping (forward and reference-mode
+ * adjoint), parallel over shot rows with OpenMP.  Built twice: REAL = double
+ * (_build/liboracle.so, the FP64 oracle) and REAL = float
+ * (_build/liboracle_f32.so, the FP32 mode: IEEE float in the same operation
+ * order as the numpy oracle's float32 mode, no contraction, so the two agree
+ * bitwise; gradient products are rounded to float and summed in double).
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/ (cross-checked against the numpy
  * oracle, oracle/fdtd_oracle.py) and by the CPU-baseline leg of bench.py.
@@ -27,32 +31,36 @@
 #include <omp.h>
 #endif
 
+#ifndef REAL
+#define REAL double
+#endif
+
 typedef struct {
     int S, NI, NJ, P;
-    double dx;
-    const double *uE0, *uE1, *uE4, *uH0, *uH1, *uH4;
-    const double* pml;
-    double *Ez, *Hx, *Hy;
-    double* psi[8];   /* Ex0, Exm, Ey0, Eym, Hx0, Hxm, Hy0, Hym */
+    REAL dx;
+    const REAL *uE0, *uE1, *uE4, *uH0, *uH1, *uH4;
+    const REAL* pml;
+    REAL *Ez, *Hx, *Hy;
+    REAL* psi[8];   /* Ex0, Exm, Ey0, Eym, Hx0, Hxm, Hy0, Hym */
 } State;
 
 #define F(a, s, i, j) (a)[((size_t)(s) * (st->NI + 1) + (i)) * (st->NJ + 1) + (j)]
 #define C(a, i, j) (a)[(size_t)(i) * (st->NJ + 1) + (j)]
 
-static int state_init(State* st, int S, int NI, int NJ, int P, double dx, const double* uE0,
-                      const double* uE1, const double* uE4, const double* uH0,
-                      const double* uH1, const double* uH4, const double* pml) {
-    st->S = S; st->NI = NI; st->NJ = NJ; st->P = P; st->dx = dx;
+static int state_init(State* st, int S, int NI, int NJ, int P, double dx, const REAL* uE0,
+                      const REAL* uE1, const REAL* uE4, const REAL* uH0,
+                      const REAL* uH1, const REAL* uH4, const REAL* pml) {
+    st->S = S; st->NI = NI; st->NJ = NJ; st->P = P; st->dx = (REAL)dx;
     st->uE0 = uE0; st->uE1 = uE1; st->uE4 = uE4;
     st->uH0 = uH0; st->uH1 = uH1; st->uH4 = uH4; st->pml = pml;
     const size_t nf = (size_t)S * (NI + 1) * (NJ + 1);
-    st->Ez = calloc(nf, sizeof(double));
-    st->Hx = calloc(nf, sizeof(double));
-    st->Hy = calloc(nf, sizeof(double));
+    st->Ez = calloc(nf, sizeof(REAL));
+    st->Hx = calloc(nf, sizeof(REAL));
+    st->Hy = calloc(nf, sizeof(REAL));
     int ok = st->Ez && st->Hx && st->Hy;
     for (int k = 0; k < 8; ++k) {
         const size_t n = (k % 4 < 2) ? (size_t)S * P * NJ : (size_t)S * NI * P;
-        st->psi[k] = calloc(n > 0 ? n : 1, sizeof(double));
+        st->psi[k] = calloc(n > 0 ? n : 1, sizeof(REAL));
         ok = ok && st->psi[k];
     }
     return ok ? 0 : -1;
@@ -67,25 +75,25 @@ static void state_free(State* st) {
  * y slabs touch Hx, so their order does not matter here). */
 static void h_phase(State* st) {
     const int S = st->S, NI = st->NI, NJ = st->NJ, P = st->P;
-    const double dx = st->dx;
+    const REAL dx = st->dx, one = (REAL)1, zero = (REAL)0;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int s = 0; s < S; ++s)
         for (int i = 0; i <= NI; ++i)
             for (int j = 0; j <= NJ; ++j) {
-                const double ez = F(st->Ez, s, i, j);
+                const REAL ez = F(st->Ez, s, i, j);
                 if (i >= 1) {
-                    const double ezj = j < NJ ? F(st->Ez, s, i, j + 1) : 0.0;
+                    const REAL ezj = j < NJ ? F(st->Ez, s, i, j + 1) : zero;
                     F(st->Hx, s, i, j) = C(st->uH0, i, j) * F(st->Hx, s, i, j) -
                                          C(st->uH1, i, j) * (ezj - ez);
                 }
                 if (j >= 1) {
-                    const double ezi = i < NI ? F(st->Ez, s, i + 1, j) : 0.0;
+                    const REAL ezi = i < NI ? F(st->Ez, s, i + 1, j) : zero;
                     F(st->Hy, s, i, j) = C(st->uH0, i, j) * F(st->Hy, s, i, j) +
                                          C(st->uH1, i, j) * (ezi - ez);
                 }
             }
     if (P == 0) return;
-    const double *RA = st->pml + 4 * P, *RB = st->pml + 5 * P, *RE = st->pml + 6 * P,
+    const REAL *RA = st->pml + 4 * P, *RB = st->pml + 5 * P, *RE = st->pml + 6 * P,
                  *RF = st->pml + 7 * P;
     /* x slabs: Hy at rows P-1-q (minus) and NI-P+q (plus) */
 #pragma omp parallel for collapse(3) schedule(static)
@@ -93,11 +101,11 @@ static void h_phase(State* st) {
         for (int s = 0; s < S; ++s)
             for (int q = 0; q < P; ++q) {
                 const int i = side == 0 ? P - 1 - q : NI - P + q;
-                double* psi = st->psi[4 + side] + ((size_t)s * P + q) * NJ;
+                REAL* psi = st->psi[4 + side] + ((size_t)s * P + q) * NJ;
                 for (int j = 0; j < NJ; ++j) {
-                    const double d = (F(st->Ez, s, i + 1, j) - F(st->Ez, s, i, j)) / dx;
+                    const REAL d = (F(st->Ez, s, i + 1, j) - F(st->Ez, s, i, j)) / dx;
                     F(st->Hy, s, i, j) = F(st->Hy, s, i, j) +
-                                         C(st->uH4, i, j) * ((RA[q] - 1.0) * d + RB[q] * psi[j]);
+                                         C(st->uH4, i, j) * ((RA[q] - one) * d + RB[q] * psi[j]);
                     psi[j] = RE[q] * psi[j] - RF[q] * d;
                 }
             }
@@ -106,12 +114,12 @@ static void h_phase(State* st) {
     for (int side = 0; side < 2; ++side)
         for (int s = 0; s < S; ++s)
             for (int i = 0; i < NI; ++i) {
-                double* psi = st->psi[6 + side] + ((size_t)s * NI + i) * P;
+                REAL* psi = st->psi[6 + side] + ((size_t)s * NI + i) * P;
                 for (int q = 0; q < P; ++q) {
                     const int j = side == 0 ? P - 1 - q : NJ - P + q;
-                    const double d = (F(st->Ez, s, i, j + 1) - F(st->Ez, s, i, j)) / dx;
+                    const REAL d = (F(st->Ez, s, i, j + 1) - F(st->Ez, s, i, j)) / dx;
                     F(st->Hx, s, i, j) = F(st->Hx, s, i, j) -
-                                         C(st->uH4, i, j) * ((RA[q] - 1.0) * d + RB[q] * psi[q]);
+                                         C(st->uH4, i, j) * ((RA[q] - one) * d + RB[q] * psi[q]);
                     psi[q] = RE[q] * psi[q] - RF[q] * d;
                 }
             }
@@ -121,7 +129,7 @@ static void h_phase(State* st) {
  * ymax (they all touch Ez, so corners see that order). */
 static void e_phase(State* st) {
     const int S = st->S, NI = st->NI, NJ = st->NJ, P = st->P;
-    const double dx = st->dx;
+    const REAL dx = st->dx, one = (REAL)1;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int s = 0; s < S; ++s)
         for (int i = 1; i <= NI; ++i)
@@ -131,7 +139,7 @@ static void e_phase(State* st) {
                      C(st->uE1, i, j) * (F(st->Hy, s, i, j) - F(st->Hy, s, i - 1, j))) -
                     C(st->uE1, i, j) * (F(st->Hx, s, i, j) - F(st->Hx, s, i, j - 1));
     if (P == 0) return;
-    const double *RA = st->pml, *RB = st->pml + P, *RE = st->pml + 2 * P, *RF = st->pml + 3 * P;
+    const REAL *RA = st->pml, *RB = st->pml + P, *RE = st->pml + 2 * P, *RF = st->pml + 3 * P;
     for (int slab = 0; slab < 4; ++slab) {   /* x0, y0, xmax, ymax */
         const int plus = slab >= 2;
         if (slab % 2 == 0) {
@@ -139,11 +147,11 @@ static void e_phase(State* st) {
             for (int s = 0; s < S; ++s)
                 for (int q = 0; q < P; ++q) {
                     const int i = plus ? NI - P + q : P - q;
-                    double* psi = st->psi[plus] + ((size_t)s * P + q) * NJ;
+                    REAL* psi = st->psi[plus] + ((size_t)s * P + q) * NJ;
                     for (int j = 0; j < NJ; ++j) {
-                        const double d = (F(st->Hy, s, i, j) - F(st->Hy, s, i - 1, j)) / dx;
+                        const REAL d = (F(st->Hy, s, i, j) - F(st->Hy, s, i - 1, j)) / dx;
                         F(st->Ez, s, i, j) = F(st->Ez, s, i, j) +
-                                             C(st->uE4, i, j) * ((RA[q] - 1.0) * d + RB[q] * psi[j]);
+                                             C(st->uE4, i, j) * ((RA[q] - one) * d + RB[q] * psi[j]);
                         psi[j] = RE[q] * psi[j] - RF[q] * d;
                     }
                 }
@@ -151,12 +159,12 @@ static void e_phase(State* st) {
 #pragma omp parallel for collapse(2) schedule(static)
             for (int s = 0; s < S; ++s)
                 for (int i = 0; i < NI; ++i) {
-                    double* psi = st->psi[2 + plus] + ((size_t)s * NI + i) * P;
+                    REAL* psi = st->psi[2 + plus] + ((size_t)s * NI + i) * P;
                     for (int q = 0; q < P; ++q) {
                         const int j = plus ? NJ - P + q : P - q;
-                        const double d = (F(st->Hx, s, i, j) - F(st->Hx, s, i, j - 1)) / dx;
+                        const REAL d = (F(st->Hx, s, i, j) - F(st->Hx, s, i, j - 1)) / dx;
                         F(st->Ez, s, i, j) = F(st->Ez, s, i, j) -
-                                             C(st->uE4, i, j) * ((RA[q] - 1.0) * d + RB[q] * psi[q]);
+                                             C(st->uE4, i, j) * ((RA[q] - one) * d + RB[q] * psi[q]);
                         psi[q] = RE[q] * psi[q] - RF[q] * d;
                     }
                 }
@@ -165,10 +173,10 @@ static void e_phase(State* st) {
 }
 
 /* Forward time loop: sample, H, E, sources (row k of scale), snapshot. */
-int oracle_forward(int S, int NI, int NJ, int P, double dx, const double* uE0, const double* uE1,
-                   const double* uE4, const double* uH0, const double* uH1, const double* uH4,
-                   const double* pml, int nsrc, const int64_t* src, const double* scale, int nrx,
-                   const int64_t* rx, int nt, double* traces, double* snaps) {
+int oracle_forward(int S, int NI, int NJ, int P, double dx, const REAL* uE0, const REAL* uE1,
+                   const REAL* uE4, const REAL* uH0, const REAL* uH1, const REAL* uH4,
+                   const REAL* pml, int nsrc, const int64_t* src, const REAL* scale, int nrx,
+                   const int64_t* rx, int nt, REAL* traces, REAL* snaps) {
     State stv, *st = &stv;
     if (state_init(st, S, NI, NJ, P, dx, uE0, uE1, uE4, uH0, uH1, uH4, pml)) {
         state_free(st);
@@ -193,7 +201,7 @@ int oracle_forward(int S, int NI, int NJ, int P, double dx, const double* uE0, c
             for (int s = 0; s < S; ++s)
                 for (int i = 0; i < NI; ++i)
                     memcpy(snaps + (((size_t)n * S + s) * NI + i) * NJ, &F(st->Ez, s, i, 0),
-                           sizeof(double) * NJ);
+                           sizeof(REAL) * NJ);
         }
     }
     state_free(st);
@@ -203,11 +211,12 @@ int oracle_forward(int S, int NI, int NJ, int P, double dx, const double* uE0, c
 /* Reference-mode adjoint: for n = nt-1 .. 1 inject -resid at the receivers
  * (in shot-major, receiver order), E, H, then accumulate
  * ge += sum_s (snap[n] - snap[n-1]) * lam, gs += sum_s snap[n] * lam
- * (shots summed in order).  ge and gs (NI, NJ) are accumulated into. */
-int oracle_adjoint(int S, int NI, int NJ, int P, double dx, const double* uE0, const double* uE1,
-                   const double* uE4, const double* uH0, const double* uH1, const double* uH4,
-                   const double* pml, int nrx, const int64_t* rx, int nt, const double* resid,
-                   const double* snaps, double* ge, double* gs) {
+ * (products in REAL, shots summed in order in double).  ge and gs (NI, NJ,
+ * double) are accumulated into. */
+int oracle_adjoint(int S, int NI, int NJ, int P, double dx, const REAL* uE0, const REAL* uE1,
+                   const REAL* uE4, const REAL* uH0, const REAL* uH1, const REAL* uH4,
+                   const REAL* pml, int nrx, const int64_t* rx, int nt, const REAL* resid,
+                   const REAL* snaps, double* ge, double* gs) {
     State stv, *st = &stv;
     if (state_init(st, S, NI, NJ, P, dx, uE0, uE1, uE4, uH0, uH1, uH4, pml)) {
         state_free(st);
@@ -222,17 +231,17 @@ int oracle_adjoint(int S, int NI, int NJ, int P, double dx, const double* uE0, c
             }
         e_phase(st);
         h_phase(st);
-        const double* sn = snaps + (size_t)n * S * NI * NJ;
-        const double* sp = snaps + (size_t)(n - 1) * S * NI * NJ;
+        const REAL* sn = snaps + (size_t)n * S * NI * NJ;
+        const REAL* sp = snaps + (size_t)(n - 1) * S * NI * NJ;
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < NI; ++i)
             for (int j = 0; j < NJ; ++j) {
                 double ae = 0.0, as = 0.0;
                 for (int s = 0; s < S; ++s) {
                     const size_t o = ((size_t)s * NI + i) * NJ + j;
-                    const double lam = F(st->Ez, s, i, j);
-                    ae += (sn[o] - sp[o]) * lam;
-                    as += sn[o] * lam;
+                    const REAL lam = F(st->Ez, s, i, j);
+                    ae += (double)((sn[o] - sp[o]) * lam);
+                    as += (double)(sn[o] * lam);
                 }
                 ge[(size_t)i * NJ + j] += ae;
                 gs[(size_t)i * NJ + j] += as;

@@ -1,13 +1,16 @@
-"""The C/OpenMP FP64 restatement (oracle/fdtd_oracle.c) against the numpy
-oracle on identical inputs: forward traces, snapshots and reference-mode
-gradients agree to rounding (different summation grouping only)."""
+"""The C/OpenMP restatement (oracle/fdtd_oracle.c) against the numpy oracle
+on identical inputs.  FP64: forward traces, snapshots and reference-mode
+gradients agree to rounding (different summation grouping only).  FP32 mode:
+the same operations in the same order in IEEE float, so traces, snapshots and
+gradient sums are bitwise the numpy float32 mode's."""
 import numpy as np
 import pytest
 
 from oracle import c_oracle as CO
 from oracle import fdtd_oracle as O
 
-pytestmark = pytest.mark.skipif(not CO.available(), reason="oracle/_build/liboracle.so not built")
+pytestmark = pytest.mark.skipif(not (CO.available() and CO.available(np.float32)),
+                                reason="oracle/_build/liboracle*.so not built")
 
 
 def _problem(NI=40, NJ=30, P=6, S=3, nt=120, nrx=5, seed=3):
@@ -52,6 +55,24 @@ def test_c_adjoint_matches_numpy():
     for a, b in ((ge_c, ge_n), (gs_c, gs_n)):
         assert np.abs(b).max() > 0
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-11 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("P", [0, 6])
+def test_c_fp32_mode_is_bitwise_numpy_fp32(P):
+    pb = _problem(P=P)
+    f = np.float32
+    c, pml = O.setup(pb, f)
+    scale = O.source_scale(pb.wavelet, pb.dx, f)
+    tr_n, sn_n = O.forward(c, pml, pb.P, pb.dx, pb.src, scale, pb.rx, pb.nt, f, True)
+    tr_c, sn_c = CO.forward(c, pml, pb.P, pb.dx, pb.src, scale, pb.rx, pb.nt, True, dtype=f)
+    assert tr_c.dtype == f and np.abs(tr_n).max() > 0
+    assert np.array_equal(tr_c, tr_n) and np.array_equal(sn_c, sn_n)
+    resid = np.random.default_rng(9).standard_normal((pb.src.shape[0], pb.nt, pb.rx.shape[1]))
+    cadj, _ = O.setup(pb, f, adjoint=True)
+    ge_n, gs_n = O.adjoint_reference(cadj, pml, pb.P, pb.dx, pb.rx, resid, sn_n, f)
+    ge_c, gs_c = CO.adjoint_reference(cadj, pml, pb.P, pb.dx, pb.rx, resid, sn_n, dtype=f)
+    assert np.abs(ge_n).max() > 0
+    assert np.array_equal(ge_c, ge_n) and np.array_equal(gs_c, gs_n)
 
 
 def test_c_oracle_reports_threads():
