@@ -56,6 +56,10 @@ def parse(argv=None):
                     help="run the serving loop under Poisson arrivals (20 requests/s) for this "
                          "many seconds of arrivals; prints its own JSON line")
     ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (requests/s)")
+    ap.add_argument("--kernel-busy", nargs=3, metavar=("DB", "T0_NS", "T1_NS"),
+                    help="read a rocprofv3 kernel-trace database and print how much of the "
+                         "monotonic window [T0, T1] (the serving line's window_ns) had a kernel "
+                         "running, with per-kernel totals; uses no GPU")
     ap.add_argument("--traffic", default=str(Path(__file__).resolve().parent / "profiles" /
                                              "traffic_c2_crosshole.json"),
                     help="measured HBM bytes per launch (from the counter runs), if present")
@@ -94,8 +98,45 @@ def kernel_bytes(kernel: int, S: int, NI: int, NJ: int, P: int, nrx: int, k: int
     raise ValueError(kernel)
 
 
+def kernel_busy(db: str, t0: int, t1: int, top: int = 10) -> dict:
+    """Union of the kernel intervals of a rocprofv3 trace inside [t0, t1]
+    (CLOCK_MONOTONIC ns, the clock of rocprofv3 timestamps and of
+    time.monotonic_ns) and per-kernel launch counts and total durations."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    iv, tot, cnt = [], {}, {}
+    for name, s, e in con.execute("select name, start, end from kernels"):
+        if s < t0 or e > t1:
+            continue
+        iv.append((s, e))
+        short = re.sub(r"\(.*", "", name.replace("(anonymous namespace)::", "")).replace("void ", "")
+        tot[short] = tot.get(short, 0) + (e - s)
+        cnt[short] = cnt.get(short, 0) + 1
+    iv.sort()
+    busy, cs, ce = 0, None, None
+    for s, e in iv:
+        if ce is None or s > ce:
+            if ce is not None:
+                busy += ce - cs
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    if ce is not None:
+        busy += ce - cs
+    kern = sorted(tot, key=tot.get, reverse=True)[:top]
+    return {"window_s": (t1 - t0) / 1e9, "busy_s": busy / 1e9,
+            "busy_fraction": busy / max(t1 - t0, 1), "launches": len(iv),
+            "kernels": {k: {"launches": cnt[k], "total_s": round(tot[k] / 1e9, 4),
+                            "avg_us": round(tot[k] / cnt[k] / 1e3, 2)} for k in kern}}
+
+
 def main():
     args = parse()
+    if args.kernel_busy:
+        db, t0, t1 = args.kernel_busy
+        print(json.dumps(kernel_busy(db, int(t0), int(t1))))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the shot batch runs on one MI355X; multi-GPU sharding is design-only).",
@@ -500,6 +541,7 @@ def serve_bench(dev, seconds, rate):
             srv.submit(r)
 
     th = threading.Thread(target=submitter, daemon=True)
+    m0 = time.monotonic_ns()
     th.start()
     last = time.perf_counter()
     while True:
@@ -515,6 +557,7 @@ def serve_bench(dev, seconds, rate):
             last = now
     torch.cuda.synchronize()
     srv.stats.finished = time.perf_counter()
+    m1 = time.monotonic_ns()
     summ = srv.stats.summary()
     # alone vs in-mix on a sample of the completed requests
     worst = 0.0
@@ -542,7 +585,9 @@ def serve_bench(dev, seconds, rate):
         "offered_gcell_updates_per_s": round(offered / seconds / 1e9, 3),
         "max_concurrent_requests": summ["max_concurrent_requests"],
         "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
-        "launches": summ["launches"],
+        "launches": summ["launches"], "ticks": summ["ticks"],
+        "host_ms_per_tick": round(summ["host_ms_per_tick"], 3),
+        "window_ns": [m0, m1],
         "alone_vs_mix_max_rel_diff": worst}), flush=True)
 
 

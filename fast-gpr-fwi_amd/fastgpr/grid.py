@@ -30,6 +30,14 @@ def _cpu(x):
     return x.detach().to("cpu") if isinstance(x, torch.Tensor) else x
 
 
+def _to_dev(t: torch.Tensor, dev) -> torch.Tensor:
+    """Host tensor -> ``dev`` without waiting for queued device work (a
+    pageable copy would): through pinned memory, asynchronously."""
+    if t.device.type == "cpu" and torch.device(dev).type == "cuda":
+        return t.contiguous().pin_memory().to(dev, non_blocking=True)
+    return t.to(dev)
+
+
 def cfl_dt(dx):
     """2-D TMz Courant limit, dy = dx (compute.py:754).  Keeps dx's type:
     a float32 tensor dx gives a float32 tensor dt, as in the reference."""
@@ -118,7 +126,7 @@ def pml_profile(P: int, dx, dt, er, mr) -> torch.Tensor:
     mr_mean = summr / n
     d = _cpu(dx)
     if isinstance(d, torch.Tensor):
-        d = d.to(dev)
+        d = _to_dev(d, dev)
     smax = (0.8 * (4 + 1)) / (Z0 * d * torch.sqrt(er_mean * mr_mean))
     L = P + 1
     tmp = (torch.linspace(0, (L - 1) + 0.5, steps=2 * L, device=dev) / (L - 1)) ** 4
@@ -128,7 +136,7 @@ def pml_profile(P: int, dx, dt, er, mr) -> torch.Tensor:
     alp = torch.zeros(P, device=dev)
     dtc = _cpu(dt)
     if isinstance(dtc, torch.Tensor):
-        dtc = dtc.to(dev)
+        dtc = _to_dev(dtc, dev)
     rows = []
     for sig in (Esig, Hsig):
         tmpd = (2 * EPS0 * kap) + dtc * (alp * kap + sig)

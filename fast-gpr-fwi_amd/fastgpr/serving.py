@@ -31,6 +31,7 @@ import numpy as np
 import torch
 
 from . import batching
+from .solver import to_device
 from .workloads import Workload
 
 
@@ -58,6 +59,8 @@ class Stats:
     launches: int = 0
     peak_bytes: int = 0
     max_active: int = 0
+    ticks: int = 0
+    tick_host_s: float = 0.0        # host time spent packing and launching ticks
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -66,7 +69,8 @@ class Stats:
                 "p50_latency_s": float(np.percentile(lat, 50)),
                 "p99_latency_s": float(np.percentile(lat, 99)),
                 "sustained_gcell_updates_per_s": self.cell_updates / wall / 1e9,
-                "wall_s": wall, "launches": self.launches,
+                "wall_s": wall, "launches": self.launches, "ticks": self.ticks,
+                "host_ms_per_tick": 1e3 * self.tick_host_s / max(self.ticks, 1),
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
 
 
@@ -140,7 +144,9 @@ class Server:
         import fastgpr
         wl = req.wl
         dev = self.device
-        t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
+        # pinned, asynchronous copies: a pageable copy would first wait for
+        # every queued tick and leave the device idle while this job is set up
+        t = lambda a: to_device(torch.from_numpy(np.ascontiguousarray(a, np.float32))[..., None], dev)
         er = torch.clamp(t(req.model[0]), min=1)
         se = torch.clamp(t(req.model[1]), min=0)
         mr = t(wl.mr)
@@ -148,7 +154,7 @@ class Server:
                                    None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
                                    er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
                                    False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
-        obs = None if req.obs is None else torch.as_tensor(req.obs).to(dev)
+        obs = None if req.obs is None else to_device(torch.as_tensor(req.obs), dev)
         req.tb, req.checkpoint = tb, C
         return batching.Job(G, er, se, mr, obs=obs, tb=tb, checkpoint=C, tag=req.rid)
 
@@ -176,7 +182,10 @@ class Server:
         busy = bool(self.sched.active())
         if busy:
             before = self.sched.launches
+            t = time.perf_counter()
             self.sched.tick()
+            self.stats.tick_host_s += time.perf_counter() - t
+            self.stats.ticks += 1
             self.stats.launches += self.sched.launches - before
             ev = torch.cuda.Event()
             ev.record()

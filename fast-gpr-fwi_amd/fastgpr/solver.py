@@ -50,9 +50,8 @@ def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device) -> 
     """5 x (NI+1) x pitch coefficient planes (Ucget / Ucgeta, uc.cu:48, 94)."""
     NI, NJ = er.shape[0], er.shape[1]
     pitch = _native.pitch(NJ)
-    e = er.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
-    s = se.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
-    m = mr.detach().reshape(NI, NJ).to(device=device, dtype=torch.float32).contiguous()
+    e, s, m = (to_device(x.detach().reshape(NI, NJ).to(torch.float32), device).contiguous()
+               for x in (er, se, mr))
     coef = torch.empty((5, NI + 1, pitch), dtype=torch.float32, device=device)
     check(lib.fgpr_cell_coeffs(_ptr(e), _ptr(s), _ptr(m), NI, NJ, dt32, dx32,
                                1 if adjoint else 0, _ptr(coef), _stream(device)), "cell_coeffs")

@@ -32,3 +32,23 @@ def test_defaults():
     import bench
     a = bench.parse([])
     assert (a.gpus, a.steps, a.warmup, a.shot_streams, a.time_block) == (1, 5, 1, 3, "auto")
+
+
+def test_kernel_busy_takes_the_union_inside_the_window(tmp_path):
+    import sqlite3
+    import bench
+    db = tmp_path / "t.db"
+    con = sqlite3.connect(db)
+    con.execute("create table kernels (name text, start integer, end integer)")
+    rows = [("void k_a(int)", 100, 200), ("void k_b(float)", 150, 250),   # overlap: 150 busy
+            ("void k_a(int)", 300, 350),                                   # 50 busy
+            ("void k_a(int)", 50, 120),                                    # starts before t0
+            ("void k_b(float)", 900, 1100)]                                # ends after t1
+    con.executemany("insert into kernels values (?, ?, ?)", rows)
+    con.commit()
+    con.close()
+    r = bench.kernel_busy(str(db), 100, 1000)
+    assert r["launches"] == 3
+    assert abs(r["busy_s"] - 200e-9) < 1e-15 and abs(r["busy_fraction"] - 200 / 900) < 1e-12
+    assert r["kernels"]["k_a"] == {"launches": 2, "total_s": round(150e-9, 4), "avg_us": 0.07}
+    assert r["kernels"]["k_b"]["launches"] == 1
