@@ -869,7 +869,12 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     // leaves them at zero; only the reference's row-0 / column-0 exceptions
     // and the CPML terms need conditions (rows: uniform per wave; columns:
     // per lane).
-    auto h_half = [&]() {
+    //
+    // XS: this wave's rows may hold row 0 or x-slab rows (row conditions and
+    // x-slab terms needed); YS: the region's columns may hold column 0 or
+    // y-slab columns.  Code for a side that cannot occur is compiled out.
+    auto h_half = [&](auto xs_tag, auto ys_tag) {
+        constexpr bool XS = decltype(xs_tag)::value, YS = decltype(ys_tag)::value;
         const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
@@ -877,17 +882,17 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             const float dj = lane_next(e) - e;
             const float di = ((m + 1 < CPT) ? ez[m + 1 < CPT ? m + 1 : m] : ez_below) - e;
             const int o = (r0 + m) * 64 + lane;
-            float vx = rge1(m) ? upd_hx(hx[m], cH1[m], dj) : hx[m];
-            float vy = (gj >= 1) ? upd_hy(hy[m], cH1[m], di) : hy[m];
-            if (yqh >= 0) {
+            float vx = (!XS || rge1(m)) ? upd_hx(hx[m], cH1[m], dj) : hx[m];
+            float vy = (!YS || gj >= 1) ? upd_hy(hy[m], cH1[m], di) : hy[m];
+            if (YS && yqh >= 0) {
                 const float4 yc = sXC[yqh];
                 const float dd = dj * idx;
                 const float ph = sPsi[2][o];
                 vx = fmaf(-cH4[m], cpml_t(yc.x, yc.y, dd, ph), vx);
                 sPsi[2][o] = cpml_psi(yc.z, yc.w, dd, ph);
             }
-            const int q = qh_m[m];
-            if (q >= 0) {
+            const int q = XS ? qh_m[m] : -1;
+            if (XS && q >= 0) {
                 const float4 xc = sXC[q];
                 const float dd = di * idx;
                 const float ph = sPsi[0][o];
@@ -902,12 +907,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 
     // General E update of owned cell m: plain update, the row-0 / column-0
     // exceptions, then the CPML terms in slab order x0, y0, xmax, ymax.
-    auto e_cell = [&](int m, int o, float hyim, float hxjm) -> float {
+    auto e_cell = [&](auto xs_tag, auto ys_tag, int m, int o, float hyim, float hxjm) -> float {
+        constexpr bool XS = decltype(xs_tag)::value, YS = decltype(ys_tag)::value;
         const float e0 = ez[m];
         const float dhy = hy[m] - hyim, dhx = hx[m] - hxjm;
         float v = upd_e(cE0[m], cE1[m], e0, dhy, dhx);
-        if (!rge1(m)) v = e0;
-        v = (gj >= 1) ? v : e0;
+        if (XS && !rge1(m)) v = e0;
+        if (YS) v = (gj >= 1) ? v : e0;
         const float c4 = cE4[m];
         auto yterm = [&]() {
             const float4 yc = sXC[kPmax + yqe];
@@ -916,9 +922,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             v = fmaf(-c4, cpml_t(yc.x, yc.y, dyd, p), v);
             sPsi[3][o] = cpml_psi(yc.z, yc.w, dyd, p);
         };
-        const int q = qe_m[m];
-        if (q < 0) {
-            if (yqe >= 0) yterm();
+        const int q = XS ? qe_m[m] : -1;
+        if (!XS || q < 0) {
+            if (YS && yqe >= 0) yterm();
         } else {
             const float dxd = dhy * idx;
             auto xterm = [&]() {
@@ -929,19 +935,23 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
             };
             if (xminus(m)) {
                 xterm();
-                if (yqe >= 0) yterm();
+                if (YS && yqe >= 0) yterm();
             } else {
-                if (yqe >= 0 && ye_lo) yterm();
+                if (YS && yqe >= 0 && ye_lo) yterm();
                 xterm();
-                if (yqe >= 0 && !ye_lo) yterm();
+                if (YS && yqe >= 0 && !ye_lo) yterm();
             }
         }
         return v;
     };
 
-    // Interior regions (no CPML cell, no domain edge, every cell updated)
-    // take a branch-free copy of the step loop.
-    const bool interior = gi0 >= P + 1 && gi0 + RI <= NI - P && gj0 >= P + 1 && gj0 + 64 <= NJ - P;
+    // A wave whose rows hold no CPML row, no row 0 and nothing outside the
+    // domain has clean rows; a region whose columns hold no CPML column, no
+    // column 0 and nothing outside the domain has clean columns.  Each wave
+    // runs the copy of the step loop compiled for its case, the branch-free
+    // one when both are clean; every copy meets the same barriers.
+    const bool rows_clean = gi0 + r0 >= P + 1 && gi0 + r0 + CPT <= NI - P;   // wave-uniform
+    const bool cols_clean = gj0 >= P + 1 && gj0 + 64 <= NJ - P;              // region-uniform
 
     auto h_fast = [&]() {
         const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
@@ -1024,8 +1034,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         }
     }
 
-    auto steps = [&](auto fast_tag) {
-        constexpr bool FAST = decltype(fast_tag)::value;
+    auto steps = [&](auto xs_tag, auto ys_tag) {
+        constexpr bool FAST = !decltype(xs_tag)::value && !decltype(ys_tag)::value;
         // forward: slot of step n; adjoint: slot of step n - 2 (the prefetch)
         int sl = ADJ ? (nsteps > 1 ? slot_of(n0 - 2) : 0) : (snap_wave ? slot_of(n0) : 0);
         for (int k = 0; k < nsteps; ++k) {
@@ -1046,7 +1056,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         }
                     }
                 }
-                if (FAST) h_fast(); else h_half();
+                if (FAST) h_fast(); else h_half(xs_tag, ys_tag);
                 __syncthreads();
                 const float hy_above = (w > 0) ? sHyBot[(w - 1) * 64 + lane] : 0.f;
 #pragma unroll
@@ -1059,7 +1069,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         v = upd_e(cE0[m], cE1[m], ez[m], hy[m] - hyim, hx[m] - hxjm);
                     } else {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
-                        v = e_cell(m, r * 64 + lane, hyim, hxjm);
+                        v = e_cell(xs_tag, ys_tag, m, r * 64 + lane, hyim, hxjm);
                     }
                     if (src_any && (srcbits & (1u << m))) {
                         for (int kk = 0; kk < d.nsrc; ++kk) {
@@ -1105,7 +1115,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         lam = upd_e(cE0[m], cE1[m], ez[m], hy[m] - hyim, hx[m] - hxjm);
                     } else {
                         const float hyim = (m > 0) ? hy[m > 0 ? m - 1 : 0] : hy_above;
-                        lam = e_cell(m, r * 64 + lane, hyim, hxjm);
+                        lam = e_cell(xs_tag, ys_tag, m, r * 64 + lane, hyim, hxjm);
                     }
                     ez[m] = lam;
                     if (r >= K && r < K + TI && centre_col) {
@@ -1133,13 +1143,20 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 sl = slot_prev(sl);
                 sEzTop[w * 64 + lane] = ez[0];
                 __syncthreads();
-                if (FAST) h_fast(); else h_half();
+                if (FAST) h_fast(); else h_half(xs_tag, ys_tag);
                 __syncthreads();
             }
         }
     };
-    if (interior) steps(std::true_type{});
-    else steps(std::false_type{});
+    using Yes = std::true_type;
+    using No = std::false_type;
+    if (rows_clean) {
+        if (cols_clean) steps(No{}, No{});
+        else steps(No{}, Yes{});
+    } else {
+        if (cols_clean) steps(Yes{}, No{});
+        else steps(Yes{}, Yes{});
+    }
 
     // ---- write back the tile centre ----
     float* pout = a.psi_out;
