@@ -31,11 +31,12 @@ from ._native import check, lib
 def interval(G, checkpoint) -> int | None:
     """Steps between saved states, or None for full snapshot storage.
 
-    "auto" checkpoints only when the full history would take more than half
-    of the free device memory, with C about sqrt(3 nt) rounded up to a whole
-    number of captured-graph chunks.  A history under a quarter of the
-    device's memory is stored without asking the device for its free
-    memory."""
+    "auto" checkpoints only when the full history would not fit in the free
+    device memory (counting what torch's caching allocator holds unused) less
+    a reserve for the rest of the gradient (5 % of the device, at least
+    8 GiB), with C about sqrt(3 nt) rounded up to a whole number of
+    captured-graph chunks.  A history under a quarter of the device's memory
+    is stored without asking the device for its free memory."""
     if checkpoint is None or checkpoint is False:
         return None
     nt = G.iterations
@@ -45,10 +46,12 @@ def interval(G, checkpoint) -> int | None:
             _, _, ti, tj = _native.tb_tiles(G.NI, G.NJ, *G.time_block)
             slot = G.S * ti * tj * G.time_block[0] * 64          # time-blocked layout
         full = nt * slot * 4
-        if full <= 0.25 * torch.cuda.get_device_properties(G.device).total_memory:
+        total = torch.cuda.get_device_properties(G.device).total_memory
+        if full <= 0.25 * total:
             return None
         free, _ = torch.cuda.mem_get_info(G.device)
-        if full <= 0.5 * free:
+        free += torch.cuda.memory_reserved(G.device) - torch.cuda.memory_allocated(G.device)
+        if full <= free - max(8 << 30, 0.05 * total):
             return None
         c = int(math.ceil(math.sqrt(3.0 * nt)))
         q = max(G.graph_chunk, 1)

@@ -23,9 +23,10 @@ Extra keyword arguments (all default to the reference behaviour):
                      interval C (keep the solver state every C steps and
                      recompute snapshots segment by segment in the adjoint),
                      or "auto" (the default: store every snapshot, as the
-                     reference, unless the history would not fit in half the
-                     free device memory; checkpointed gradients are bitwise
-                     the stored ones).
+                     reference, unless the history would not fit in the free
+                     device memory less a reserve for the rest of the
+                     gradient; checkpointed gradients are bitwise the stored
+                     ones).
 * ``shot_streams`` -- shot groups of a time-blocked sweep, each on its own
                      HIP stream so that the launches of one fill the CUs the
                      other's launch tail leaves idle (default 3; 1 = one

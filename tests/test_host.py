@@ -120,3 +120,29 @@ def test_shot_groups_are_held_to_spare_hardware_queues(monkeypatch):
     assert solver.shot_groups(G(shot_streams=3)) == 1
     monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8")
     assert solver.shot_groups(G(shot_streams=5)) == 5
+
+
+def test_auto_checkpoint_counts_cached_memory_and_keeps_a_reserve(monkeypatch):
+    """checkpoint="auto" stores the history when it fits in the free memory
+    plus what torch's allocator caches unused, less the reserve; otherwise it
+    checkpoints (device queries stubbed: no GPU here)."""
+    from types import SimpleNamespace
+    import torch
+    from fastgpr import _native, checkpoint as ck
+    GiB = 1 << 30
+    monkeypatch.setattr(torch.cuda, "get_device_properties",
+                        lambda d: SimpleNamespace(total_memory=288 * GiB))
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (100 * GiB, 288 * GiB))
+    monkeypatch.setattr(torch.cuda, "memory_allocated", lambda d: 10 * GiB)
+    G = SimpleNamespace(iterations=1500, S=13, NI=1600, NJ=800, time_block=None, options={},
+                        graph_chunk=32, device="cuda:0")
+    full = 1500 * 13 * 1600 * _native.pitch(800) * 4
+    assert 72 * GiB < full < 100 * GiB
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 150 * GiB)
+    assert ck.interval(G, "auto") is None                 # 100 + 140 cached - 14.4 reserve
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 10 * GiB)
+    C = ck.interval(G, "auto")                             # 100 - 14.4 < full
+    assert C is not None and C % 32 == 0 and C >= int(np.ceil(np.sqrt(3 * 1500)))
+    G.NI = 400                                             # a quarter of the device: no query
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (_ for _ in ()).throw(AssertionError))
+    assert ck.interval(G, "auto") is None
