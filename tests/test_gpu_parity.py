@@ -110,6 +110,30 @@ def test_time_blocked_matches_per_step_kernels(hip_device, tb):
         assert rel_l2(gb[k], ga[k]) < 1e-12
 
 
+@pytest.mark.parametrize("tb", [(64, 1024, 6), (32, 512, 4), (64, 512, 3)])
+def test_time_blocked_step_loop_copies_match_per_step_kernels(hip_device, tb):
+    """200 x 160 cells with a 10-cell CPML: some regions have clean columns
+    (no CPML column, no column 0) and some waves clean rows, so all four
+    compiled copies of the step loop run (clean rows and columns, dirty
+    rows only, dirty columns only, both dirty) in one launch; traces are
+    bitwise those of the per-step kernels."""
+    from fastgpr import _native
+    from fastgpr.workloads import small_case
+    wl = small_case(NI=200, NJ=160, P=10, S=2, nt=61, nrx=5, seed=8)
+    RI, _, K = tb
+    TJ = 64 - 2 * K - 1
+    starts = [tj * TJ - K for tj in range(_native.tb_tiles(200, 160, *tb)[3])]
+    assert any(g >= wl.P + 1 and g + 64 <= wl.NJ - wl.P for g in starts)     # clean columns
+    assert any(g < wl.P + 1 or g + 64 > wl.NJ - wl.P for g in starts)        # CPML columns
+    resid = _resid(wl)
+    print("variant", tb, flush=True)
+    a, ga = run_product(wl, hip_device, resid=resid)
+    b, gb = run_product(wl, hip_device, resid=resid, time_block=tb, graph_chunk=8)
+    assert np.array_equal(a, b)
+    for k in ("er", "se"):
+        assert rel_l2(gb[k], ga[k]) < 1e-12
+
+
 @pytest.mark.parametrize("groups", [2, 3, 5])
 def test_shot_groups_on_streams_are_bitwise_one_batch(hip_device, groups):
     """Shot groups swept on separate streams give the traces and the
