@@ -9,5 +9,6 @@ ctypes on import; importing fails if the library has not been built.
 """
 from . import _native  # noqa: F401  (loads libfastgpr.so or raises ImportError)
 from .compute import GPRFWI, GPRFWIModel, compute, initialization  # noqa: F401
+from .solver import release_graphs  # noqa: F401
 
-__all__ = ["compute", "GPRFWI", "GPRFWIModel", "initialization"]
+__all__ = ["compute", "GPRFWI", "GPRFWIModel", "initialization", "release_graphs"]

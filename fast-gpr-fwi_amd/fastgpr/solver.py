@@ -304,6 +304,17 @@ def max_shot_groups() -> int:
 _sweep_log = None
 
 
+def release_graphs():
+    """Free every captured graph the library holds.  Graphs are cached by
+    their descriptor (device pointers and sizes), so a caller that keeps its
+    buffers in place reuses them; one whose buffers move (a sweep over many
+    shapes) accumulates them and can call this between calls.  Waits for all
+    device work first, so no released graph is still running."""
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+    _native.check(_native.lib.fgpr_release_graphs(), "release graphs")
+
+
 def record_sweeps(on: bool):
     """Start recording a pair of HIP events on the calling stream around
     every full-history sweep (forward and reference adjoint: from before the

@@ -281,7 +281,10 @@ int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* strea
 int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, float* out,
              void* stream);
 
-/* Release every HIP graph the library captured (also done at unload). */
+/* Release every HIP graph the library captured.  Graphs are cached by the
+ * descriptor's bytes (device pointers and sizes), so callers whose buffers move
+ * accumulate them; call this only when none of them is still running (after
+ * synchronising the device). */
 int fgpr_release_graphs(void);
 
 #ifdef __cplusplus

@@ -148,6 +148,21 @@ def test_shot_groups_on_streams_are_bitwise_one_batch(hip_device, groups):
     assert np.array_equal(ga["er"], gb["er"]) and np.array_equal(ga["se"], gb["se"])
 
 
+def test_released_graphs_are_captured_again(hip_device):
+    """fastgpr.release_graphs() frees the cached graphs; the next call
+    captures new ones and gives the same traces and gradients bitwise."""
+    import fastgpr
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=90)
+    resid = _resid(wl)
+    a, ga = run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6))
+    fastgpr.release_graphs()
+    b, gb = run_product(wl, hip_device, resid=resid, time_block=(64, 1024, 6))
+    assert np.array_equal(a, b)
+    for k in ("er", "se"):
+        assert np.array_equal(ga[k], gb[k])
+
+
 def test_time_blocked_eager_equals_graph(hip_device):
     from fastgpr.workloads import small_case
     wl = small_case(NI=50, NJ=130, nt=90)
