@@ -622,9 +622,9 @@ struct TBArgs {
 // Work order of a launch.  Workgroups are dispatched round-robin over the 8
 // XCDs (block b to XCD b % 8, in order b / 8 within it).  Each XCD gets an
 // equal share of the interior items first and then an equal share of the
-// general (CPML / edge) items, which cost about 1.6x as much: the cheap ones
-// finish early and free their CUs for the expensive ones, instead of an
-// expensive tail.  An item is (tile, shot), shots innermost, so the shots of
+// general (CPML / edge) items, which cost more (their waves run the CPML and
+// edge copies of the step loop): the cheap ones finish early and free their
+// CUs for the expensive ones, instead of an expensive tail.  An item is (tile, shot), shots innermost, so the shots of
 // a tile (which share coefficients) meet in one XCD's L2.  Returns the
 // shot and the tile of block `wg` (a bijection over [0, nwg)).
 __device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg, int& s, int& tile) {
