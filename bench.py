@@ -441,32 +441,54 @@ def _job_for(wl, dev, tb, checkpoint):
 
 def uneven_bench(dev):
     """Uneven batch: 4 models of different sizes and step counts, 64 shots,
-    advanced together by multi-problem launches (one per direction per tick),
+    advanced together by multi-problem launches (one per direction and
+    variant per tick), each model with the time-blocking variant the
+    autotuner picks for its grid among those the multi-problem kernel has,
     against the same gradients run one model at a time through ``compute``
-    (time-blocked, captured graphs).  Checkpoint recompute is not counted."""
+    (autotuned time-blocked kernels, captured graphs, shot streams).
+    Checkpoint recompute is not counted."""
     import numpy as np
     import torch
     import fastgpr
-    from fastgpr.batching import Scheduler
+    from fastgpr import autotune
+    from fastgpr.batching import MULTI_VARIANTS, Scheduler, job_bytes
     from fastgpr.workloads import c3_uneven
     wls = c3_uneven()
-    tb = (64, 1024, 6)
+    # Per grid, the variant with the best throughput when the device is full
+    # (in a multi-problem launch other jobs fill a job's launch tail): tuned
+    # on the grid's acquisition repeated to at least 64 shots.
+    tbs = []
+    for w in wls:
+        t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
+        er, se, mr = torch.clamp(t(w.er0), min=1), torch.clamp(t(w.se0), min=0), t(w.mr)
+        rep = -(-64 // w.S)
+        G = fastgpr.initialization(dev, w.dx, w.NJ, w.NI, 1, w.time_window, ['hertzian', 'z'],
+                                   None, None, None, None, w.P, ['ricker', 1, w.freq], w.S * rep,
+                                   er, se, mr, 1, w.rx.shape[1], w.dt, 'HORIPML', None, None,
+                                   False, False, None, torch.tensor(np.tile(w.src, (rep, 1, 1))),
+                                   torch.tensor(np.tile(w.rx, (rep, 1, 1))), shot_streams=1)
+        tbs.append(autotune.choose(G, er, se, mr, candidates=MULTI_VARIANTS))
     torch.cuda.empty_cache()
     free, _ = torch.cuda.mem_get_info(dev)
-    # Checkpoint what compute(checkpoint="auto") checkpoints (a history over
-    # half the free memory), then the largest remaining ones until all fit.
-    from fastgpr.batching import job_bytes
-    full = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, w.nt) for w in wls]
-    ck = [b > 0.5 * free for b in full]
-    small = lambda k: job_bytes(wls[k].NI, wls[k].NJ, wls[k].S, wls[k].nt, wls[k].rx.shape[1],
-                                wls[k].P, 144)
-    while sum(small(k) if ck[k] else full[k] for k in range(len(wls))) > 0.8 * free:
-        k = max((k for k in range(len(wls)) if not ck[k]), key=lambda k: full[k])
+    # Checkpoint until every job fits in 80 % of the free memory, choosing
+    # first the jobs whose recompute (one more forward sweep) costs the
+    # fewest cell updates per byte the checkpointing frees.
+    full = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, w.nt, tb=tb)
+            for w, tb in zip(wls, tbs)]
+    small = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, 144, tb=tb)
+             for w, tb in zip(wls, tbs)]
+    ck = [False] * len(wls)
+    order = sorted(range(len(wls)),
+                   key=lambda k: wls[k].S * wls[k].NI * wls[k].NJ * wls[k].nt
+                   / max(full[k] - small[k], 1))
+    for k in order:
+        if sum(small[i] if ck[i] else full[i] for i in range(len(wls))) <= 0.8 * free:
+            break
         ck[k] = True
     cu = sum(w.cell_updates_per_gradient() for w in wls)
 
     def batched():
-        jobs = [_job_for(w, dev, tb, c) for w, c in zip(wls, ck)]
+        jobs = [_job_for(w, dev, tb, c) for w, tb, c in zip(wls, tbs, ck)]
         sch = Scheduler(dev)
         for j, _ in jobs:
             sch.add(j)
@@ -489,7 +511,7 @@ def uneven_bench(dev):
                                       ['ricker', 1, w.freq], w.S, er, se, t(w.mr), 1,
                                       w.rx.shape[1], time_windows=w.time_window, dt=w.dt,
                                       pmlthick=w.P, tv=False, customsource=torch.tensor(w.src),
-                                      customreceiver=torch.tensor(w.rx), time_block=tb,
+                                      customreceiver=torch.tensor(w.rx), time_block="auto",
                                       checkpoint="auto")
             (0.5 * (pred ** 2).sum()).backward()
         torch.cuda.synchronize()
@@ -504,7 +526,7 @@ def uneven_bench(dev):
         "one_model_at_a_time": {"value": round(cu / el_s / 1e9, 3), "seconds": round(el_s, 3)},
         "speedup": round(el_s / el_b, 3),
         "config": {"workload": "4 models, 64 shots: " + ", ".join(w.name + f"/{w.nt}" for w in wls),
-                   "cell_updates": cu, "time_block": list(tb),
+                   "cell_updates": cu, "time_block": [list(tb) for tb in tbs],
                    "checkpoint_interval": ckpts}}), flush=True)
 
 

@@ -80,6 +80,7 @@ class Job:
         self.pf_plain = self.fb.problem(self.coef_f, srcscale=self.scale)   # no snapshots
         self.result = None
         self.done_event = None
+        self.stream_idx = 0
         self.ip = 0
         self.n = 0
         self.parity = 0
@@ -222,14 +223,34 @@ class Job:
 
 class Scheduler:
     """Advances a set of jobs together: per tick, one multi-problem launch per
-    (variant, direction) group holding one block of every active job."""
+    (stream, variant, direction) group holding one block of every active job
+    of that group.
 
-    def __init__(self, device):
+    Jobs are spread over side streams (by default as many as have a hardware
+    queue of their own, at most three), each new job on the stream with the
+    least active work, so that one stream's launch tail is filled by another
+    stream's launch.  Every operation of a job runs on its stream, ordered
+    after the work queued on the calling stream when the job was added; a
+    job's ``done_event`` is recorded on its stream, and ``join`` (the end of
+    ``run``) makes the calling stream wait for all of them.  A job's
+    arithmetic does not depend on its stream or on the other jobs."""
+
+    def __init__(self, device, streams: int | None = None):
         self.device = torch.device(device)
         self.jobs: list[Job] = []
         self.launches = 0
+        cap = _s.max_shot_groups()
+        n = cap if streams is None else max(1, min(int(streams), cap))
+        self.streams = _s._side_streams(self.device, n)
 
     def add(self, job: Job):
+        load = [0] * len(self.streams)
+        for j in self.active():
+            load[j.stream_idx] += j.cell_updates()
+        job.stream_idx = min(range(len(self.streams)), key=lambda i: load[i])
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.device))
+        self.streams[job.stream_idx].wait_event(ready)
         self.jobs.append(job)
 
     def active(self):
@@ -237,30 +258,43 @@ class Scheduler:
 
     def tick(self) -> int:
         """One launch per group; returns the number of blocks advanced."""
-        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         groups: dict = {}
         for j in self.active():
             blk = j.next_block()
             if blk is None:
                 continue
             adj, item = blk
-            groups.setdefault((j.tb, adj), []).append((j, item))
-        for (tb, adj), members in groups.items():
-            n = len(members)
-            items = (_native.TBItem * n)(*[m[1] for m in members])
-            nbytes = lib.fgpr_tb_table_bytes(n)
-            host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-            total = ctypes.c_int()
-            check(lib.fgpr_tb_pack(items, n, 1 if adj else 0, ctypes.c_void_p(host.data_ptr()),
-                                   nbytes, ctypes.byref(total)), "pack")
-            dev_table = host.to(self.device, non_blocking=True)
-            check(lib.fgpr_tb_launch_table(ctypes.c_void_p(dev_table.data_ptr()), n, total.value,
-                                           tb[0], tb[1], tb[2], 1 if adj else 0, stream), "launch")
-            self.launches += 1
-            for j, _ in members:
-                j.after_block(stream)
+            groups.setdefault((j.stream_idx, j.tb, adj), []).append((j, item))
+        for (si, tb, adj), members in groups.items():
+            st = self.streams[si]
+            with torch.cuda.stream(st):
+                stream = ctypes.c_void_p(st.cuda_stream)
+                n = len(members)
+                items = (_native.TBItem * n)(*[m[1] for m in members])
+                nbytes = lib.fgpr_tb_table_bytes(n)
+                host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+                total = ctypes.c_int()
+                check(lib.fgpr_tb_pack(items, n, 1 if adj else 0,
+                                       ctypes.c_void_p(host.data_ptr()), nbytes,
+                                       ctypes.byref(total)), "pack")
+                dev_table = host.to(self.device, non_blocking=True)
+                check(lib.fgpr_tb_launch_table(ctypes.c_void_p(dev_table.data_ptr()), n,
+                                               total.value, tb[0], tb[1], tb[2],
+                                               1 if adj else 0, stream), "launch")
+                self.launches += 1
+                for j, _ in members:
+                    j.after_block(stream)
         return sum(len(m) for m in groups.values())
+
+    def join(self):
+        """The calling stream waits for the work queued on every stream."""
+        main = torch.cuda.current_stream(self.device)
+        for st in self.streams:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            main.wait_event(ev)
 
     def run(self):
         while self.active():
             self.tick()
+        self.join()

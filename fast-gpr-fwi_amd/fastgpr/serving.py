@@ -187,11 +187,15 @@ class Server:
             self.stats.tick_host_s += time.perf_counter() - t
             self.stats.ticks += 1
             self.stats.launches += self.sched.launches - before
-            ev = torch.cuda.Event()
-            ev.record()
-            self._tick_events.append(ev)
+            evs = []
+            for st in self.sched.streams:
+                ev = torch.cuda.Event()
+                ev.record(st)
+                evs.append(ev)
+            self._tick_events.append(evs)
             if len(self._tick_events) > self.max_inflight:
-                self._tick_events.pop(0).synchronize()     # bound how far the host runs ahead
+                for ev in self._tick_events.pop(0):        # bound how far the host runs ahead
+                    ev.synchronize()
         self._reap()
         return busy or bool(self.active) or bool(self.waiting) or not self.pending.empty()
 
@@ -205,6 +209,7 @@ class Server:
                 if stop_when_idle:
                     break
                 time.sleep(0.0005)
+        self.sched.join()
         self.stats.finished = time.perf_counter()
         return self.stats.summary()
 
