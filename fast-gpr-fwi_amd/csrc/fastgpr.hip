@@ -1044,6 +1044,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 // receivers of this wave, sampled before the step
 #pragma unroll
                 for (int c = 0; c < kWaveList / 64; ++c) {
+                    if (c * 64 >= wn) break;                 // wave-uniform: no receivers left
                     const int cnt = wn - c * 64 < 64 ? wn - c * 64 : 64;
                     for (int q = 0; q < cnt; ++q) {
                         const int ent = __builtin_amdgcn_readlane(rxent[c], q);
@@ -1091,6 +1092,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                 // applied by the owner of the cell in list order
 #pragma unroll
                 for (int c = 0; c < kWaveList / 64; ++c) {
+                    if (c * 64 >= wn) break;                 // wave-uniform: no receivers left
                     const int cnt = wn - c * 64 < 64 ? wn - c * 64 : 64;
                     const int mine = rxent[c];
                     const float gv = lane < cnt ? sRes[(w * K + k) * kWaveList + c * 64 + lane] : 0.f;
@@ -1118,7 +1120,10 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         lam = e_cell(xs_tag, ys_tag, m, r * 64 + lane, hyim, hxjm);
                     }
                     ez[m] = lam;
-                    if (r >= K && r < K + TI && centre_col) {
+                    // Rows outside the tile centre are skipped (a wave-uniform
+                    // test); columns are not tested: lanes outside the centre
+                    // hold zero snapshots and their sums are never written back.
+                    if (r >= K && r < K + TI) {
                         const float fn = fcur[m];
                         const float fp = fprv[m];
                         ae[m] += (double)((fn - fp) * lam);
