@@ -602,6 +602,32 @@ constexpr int kTBCols = 64;
 constexpr int kPmax = 128;
 constexpr int kWaveList = 128;   // receiver entries one wave can hold
 
+// Exact unsigned division by a launch-constant d through a multiply and a
+// shift (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up variant with the add): for
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (umulhi(n, m) + n) >> l for every 0 <= n < 2^31.  Replaces the
+// ~20-instruction integer-division sequences of the work-item mapping.
+struct FastDiv {
+    unsigned m;
+    int l;
+};
+
+inline FastDiv make_fastdiv(int d) {
+    if (d < 1) d = 1;   // divisors that can be 0 are never divided by (see tb_item)
+    int l = 0;
+    while ((1u << l) < (unsigned)d) ++l;
+    FastDiv f;
+    f.m = (unsigned)(((uint64_t)1 << 32) * (((uint64_t)1 << l) - (uint64_t)d) / (uint64_t)d + 1);
+    f.l = l;
+    return f;
+}
+
+__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+    const unsigned t = __umulhi((unsigned)n, f.m);
+    return (int)(((uint64_t)t + (unsigned)n) >> f.l);
+}
+
 struct TBArgs {
     Dev d;
     const float* Ez_in;
@@ -614,6 +640,7 @@ struct TBArgs {
     float* psi_out;
     int ti_count, tj_count;
     int int_ti0, int_ni, int_tj0, int_nj;   // rectangle of interior tiles (no CPML cell, no edge)
+    FastDiv fd_S, fd_nj, fd_per, fd_tj;     // S, int_nj, tj_count - int_nj, tj_count
     const int* rx_ptr;   // CSR over S*ntiles: receivers sampled (forward) / injected (adjoint)
     const int* rx_idx;   // entries s*nrx + r
     double* gshot;       // adjoint: S x 2 x NI x NJ per-shot partial gradients (shot-major)
@@ -638,13 +665,14 @@ __device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg,
     const int ia = x * qi + (x < ri ? x : ri);       // interior items of XCDs < x
     const int icnt = qi + (x < ri ? 1 : 0);
     if (t < icnt) {
-        const int k = ia + t, kt = k / S;
+        const int k = ia + t, kt = fdiv(k, a.fd_S);
         s = k - kt * S;
-        const int ti = a.int_ti0 + kt / nj;
-        tile = ti * tjc + a.int_tj0 + (kt - (kt / nj) * nj);
+        const int q2 = fdiv(kt, a.fd_nj);
+        const int ti = a.int_ti0 + q2;
+        tile = ti * tjc + a.int_tj0 + (kt - q2 * nj);
         return;
     }
-    const int g = (base - ia) + (t - icnt), gt = g / S;
+    const int g = (base - ia) + (t - icnt), gt = fdiv(g, a.fd_S);
     s = g - gt * S;
     // general tiles in row-major order with the interior rectangle skipped
     const int before = a.int_ti0 * tjc;              // rows above the rectangle
@@ -652,7 +680,8 @@ __device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg,
     if (ni == 0 || gt < before) { tile = gt; return; }
     const int u = gt - before;
     if (u < ni * per) {
-        const int ti = a.int_ti0 + u / per, c = u - (u / per) * per;
+        const int q3 = fdiv(u, a.fd_per);
+        const int ti = a.int_ti0 + q3, c = u - q3 * per;
         tile = ti * tjc + (c < a.int_tj0 ? c : c + nj);
         return;
     }
@@ -687,7 +716,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     const int ntiles = a.ti_count * a.tj_count;
     int s, tile;
     tb_item(a, d.S, wg, nwg, s, tile);
-    const int ti = tile / a.tj_count, tj = tile - (tile / a.tj_count) * a.tj_count;
+    const int ti = fdiv(tile, a.fd_tj), tj = tile - ti * a.tj_count;
     const int gi0 = ti * TI - K, gj0 = tj * TJ - K;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1422,6 +1451,10 @@ TBArgs make_tb(const fgpr_problem& p, const Dev& d, int parity, bool adj) {
     a.rx_ptr = adj ? p.tb_inj_ptr : p.tb_rx_ptr;
     a.rx_idx = adj ? p.tb_inj_idx : p.tb_rx_idx;
     a.gshot = p.gshot;
+    a.fd_S = make_fastdiv(p.S);
+    a.fd_nj = make_fastdiv(a.int_nj);
+    a.fd_per = make_fastdiv(a.tj_count - a.int_nj);
+    a.fd_tj = make_fastdiv(a.tj_count);
     return a;
 }
 
