@@ -623,6 +623,24 @@ inline FastDiv make_fastdiv(int d) {
     return f;
 }
 
+// Late reads of rarely used launch arguments (receiver and source tables,
+// output pointers).  `reload` returns the address the arguments live at
+// (the kernel-argument segment, or a multi-problem table entry) with its
+// provenance hidden from the optimiser by an empty asm (no instruction, no
+// memory access), and `ld_arg` reads one field through the constant address
+// space (a scalar load) where it is used, so these fields are not held in
+// scalar registers (and spilled) across the whole kernel.
+template <class T>
+__device__ __forceinline__ const T* reload(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+template <class T>
+__device__ __forceinline__ T ld_arg(const T* field) {
+    return *(const __attribute__((address_space(4))) T*)field;
+}
+
 __device__ __forceinline__ int fdiv(int n, FastDiv f) {
     const unsigned t = __umulhi((unsigned)n, f.m);
     return (int)(((uint64_t)t + (unsigned)n) >> f.l);
@@ -703,8 +721,8 @@ __device__ __forceinline__ float lane_prev(float v) {   // value of lane - 1 (0 
 }
 
 template <int RI, int NT, int K, bool ADJ>
-__device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int nsteps,
-                                        const int wg, const int nwg) {
+__device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const int n0,
+                                        const int nsteps, const int wg, const int nwg) {
     constexpr int W = NT / 64;
     constexpr int CPT = RI / W;          // rows owned by one thread (a contiguous block)
     constexpr int TI = RI - 2 * K - 1;
@@ -1082,7 +1100,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                             float v = 0.f;
 #pragma unroll
                             for (int m = 0; m < CPT; ++m) v = (m == me) ? ez[m] : v;
-                            d.traces[((size_t)s * d.nt + n) * d.nrx + (ent & 0xFFFFF)] = v;
+                            const TBArgs* ar = reload(ap);
+                            float* const tr = ld_arg(&ar->d.traces);
+                            tr[((size_t)s * ld_arg(&ar->d.nt) + n) * ld_arg(&ar->d.nrx) + (ent & 0xFFFFF)] = v;
                         }
                     }
                 }
@@ -1102,9 +1122,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
                         v = e_cell(xs_tag, ys_tag, m, r * 64 + lane, hyim, hxjm);
                     }
                     if (src_any && (srcbits & (1u << m))) {
-                        for (int kk = 0; kk < d.nsrc; ++kk) {
-                            const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
-                            if (sc[0] == gi && sc[1] == gj) v = fmaf(-cE4[m], d.srcscale[(size_t)kk * d.nt + n], v);
+                        const TBArgs* ar = reload(ap);
+                        const int ns = ld_arg(&ar->d.nsrc), snt = ld_arg(&ar->d.nt);
+                        const int* const src = ld_arg(&ar->d.src);
+                        const float* const ssc = ld_arg(&ar->d.srcscale);
+                        for (int kk = 0; kk < ns; ++kk) {
+                            const int* sc = src + 2 * ((size_t)s * ns + kk);
+                            if (sc[0] == gi && sc[1] == gj) v = fmaf(-cE4[m], ssc[(size_t)kk * snt + n], v);
                         }
                     }
                     ez[m] = v;
@@ -1193,7 +1217,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
     }
 
     // ---- write back the tile centre ----
-    float* pout = a.psi_out;
+    // output pointers read here, not held through the steps
+    const TBArgs* ar = reload(ap);
+    float* const pout = ld_arg(&ar->psi_out);
+    float* const Ez_out = ld_arg(&ar->Ez_out);
+    float* const Hx_out = ld_arg(&ar->Hx_out);
+    float* const Hy_out = ld_arg(&ar->Hy_out);
+    double* const gshot = ADJ ? ld_arg(&ar->gshot) : nullptr;
     // adjoint: read every partial-gradient pair first (one round trip), add
     // after the field stores are issued
     double g0v[ADJ ? CPT : 1], g1v[ADJ ? CPT : 1];
@@ -1202,7 +1232,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         for (int m = 0; m < CPT; ++m) {
             const int r = r0 + m, gi = gi0 + r;
             const bool own = r >= K && r < K + TI && centre_col && (act & (1u << m));
-            const double* gp = a.gshot + ((size_t)2 * s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
+            const double* gp = gshot + ((size_t)2 * s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
             g0v[m] = own ? gp[0] : 0.0;
             g1v[m] = own ? gp[(size_t)NI * NJ] : 0.0;
         }
@@ -1213,9 +1243,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (!(r >= K && r < K + TI && centre_col)) continue;
         if (!(colx && gi >= 0 && gi <= NI)) continue;
         const size_t g = shot_off + (size_t)gi * pitch + gj;
-        a.Ez_out[g] = ez[m];
-        a.Hx_out[g] = hx[m];
-        a.Hy_out[g] = hy[m];
+        Ez_out[g] = ez[m];
+        Hx_out[g] = hx[m];
+        Hy_out[g] = hy[m];
         if (!(act & (1u << m))) continue;
         const int o = r * 64 + lane;
         const int qh = P > 0 ? xq_h(gi) : -1;
@@ -1225,7 +1255,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
         if (yqh >= 0) pout[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] = sPsi[2][o];
         if (yqe >= 0) pout[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] = sPsi[3][o];
         if constexpr (ADJ) {
-            double* g0 = a.gshot + ((size_t)2 * s * NI + gi) * NJ + gj;
+            double* g0 = gshot + ((size_t)2 * s * NI + gi) * NJ + gj;
             g0[0] = g0v[m] + ae[m];
             g0[(size_t)NI * NJ] = g1v[m] + as_[m];
         }
@@ -1235,7 +1265,9 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const int n0, const int
 template <int RI, int NT, int K, bool ADJ>
 __global__ void __launch_bounds__(NT)
 k_tb(TBArgs a, int koff, int dir, int nsteps) {
-    tb_body<RI, NT, K, ADJ>(a, step_index(a.d, koff, dir), nsteps, blockIdx.x,
+    // `a` is the first kernel argument: at offset 0 of the argument segment
+    const TBArgs* ap = (const TBArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    tb_body<RI, NT, K, ADJ>(a, ap, step_index(a.d, koff, dir), nsteps, blockIdx.x,
                             a.ti_count * a.tj_count * a.d.S);
 }
 
@@ -1265,7 +1297,7 @@ k_tb_multi(const unsigned char* __restrict__ table, int nitems) {
     }
     const MultiEntry* ents = (const MultiEntry*)(table + tb_table_header(nitems));
     const MultiEntry& me = ents[lo];
-    tb_body<RI, NT, K, ADJ>(me.a, me.n0, me.nsteps, b - prefix[lo], prefix[lo + 1] - prefix[lo]);
+    tb_body<RI, NT, K, ADJ>(me.a, &me.a, me.n0, me.nsteps, b - prefix[lo], prefix[lo + 1] - prefix[lo]);
 }
 
 // Row-major Ez (S x (NI+1) x pitch) -> one slot of the time-blocked
