@@ -1217,13 +1217,18 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     }
 
     // ---- write back the tile centre ----
-    // output pointers read here, not held through the steps
+    // output pointers and the sizes the write-back indexes with, read here:
+    // nothing of the write-back is held in scalar registers through the steps
     const TBArgs* ar = reload(ap);
     float* const pout = ld_arg(&ar->psi_out);
     float* const Ez_out = ld_arg(&ar->Ez_out);
     float* const Hx_out = ld_arg(&ar->Hx_out);
     float* const Hy_out = ld_arg(&ar->Hy_out);
     double* const gshot = ADJ ? ld_arg(&ar->gshot) : nullptr;
+    const int wNI = ld_arg(&ar->d.NI), wNJ = ld_arg(&ar->d.NJ), wP = ld_arg(&ar->d.P);
+    const int wS = ld_arg(&ar->d.S), wpitch = ld_arg(&ar->d.pitch);
+    const size_t wxs = (size_t)wS * wP * wNJ, wys = (size_t)wS * wNI * wP;
+    const size_t wshot = (size_t)s * (size_t)(wNI + 1) * wpitch;
     // adjoint: read every partial-gradient pair first (one round trip), add
     // after the field stores are issued
     double g0v[ADJ ? CPT : 1], g1v[ADJ ? CPT : 1];
@@ -1232,32 +1237,33 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
         for (int m = 0; m < CPT; ++m) {
             const int r = r0 + m, gi = gi0 + r;
             const bool own = r >= K && r < K + TI && centre_col && (act & (1u << m));
-            const double* gp = gshot + ((size_t)2 * s * NI + (own ? gi : 0)) * NJ + (own ? gj : 0);
+            const double* gp = gshot + ((size_t)2 * s * wNI + (own ? gi : 0)) * wNJ + (own ? gj : 0);
             g0v[m] = own ? gp[0] : 0.0;
-            g1v[m] = own ? gp[(size_t)NI * NJ] : 0.0;
+            g1v[m] = own ? gp[(size_t)wNI * wNJ] : 0.0;
         }
     }
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int r = r0 + m, gi = gi0 + r;
         if (!(r >= K && r < K + TI && centre_col)) continue;
-        if (!(colx && gi >= 0 && gi <= NI)) continue;
-        const size_t g = shot_off + (size_t)gi * pitch + gj;
+        if (!(colx && gi >= 0 && gi <= wNI)) continue;
+        const size_t g = wshot + (size_t)gi * wpitch + gj;
         Ez_out[g] = ez[m];
         Hx_out[g] = hx[m];
         Hy_out[g] = hy[m];
         if (!(act & (1u << m))) continue;
         const int o = r * 64 + lane;
-        const int qh = P > 0 ? xq_h(gi) : -1;
-        const int qe = P > 0 ? xq_e(gi) : -1;
-        if (qh >= 0) pout[(size_t)(gi < P ? HX0 : HXM) * xs + ((size_t)s * P + qh) * NJ + gj] = sPsi[0][o];
-        if (qe >= 0) pout[(size_t)(gi <= P ? EX0 : EXM) * xs + ((size_t)s * P + qe) * NJ + gj] = sPsi[1][o];
-        if (yqh >= 0) pout[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] = sPsi[2][o];
-        if (yqe >= 0) pout[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] = sPsi[3][o];
+        // x-slab indices as in the set-up (xq_h, xq_e)
+        const int qh = wP > 0 ? (gi < wP ? wP - 1 - gi : (gi >= wNI - wP ? gi - (wNI - wP) : -1)) : -1;
+        const int qe = wP > 0 ? ((gi >= 1 && gi <= wP) ? wP - gi : (gi >= wNI - wP ? gi - (wNI - wP) : -1)) : -1;
+        if (qh >= 0) pout[(size_t)(gi < wP ? HX0 : HXM) * wxs + ((size_t)s * wP + qh) * wNJ + gj] = sPsi[0][o];
+        if (qe >= 0) pout[(size_t)(gi <= wP ? EX0 : EXM) * wxs + ((size_t)s * wP + qe) * wNJ + gj] = sPsi[1][o];
+        if (yqh >= 0) pout[4 * wxs + (size_t)(yh_lo ? 2 : 3) * wys + ((size_t)s * wNI + gi) * wP + yqh] = sPsi[2][o];
+        if (yqe >= 0) pout[4 * wxs + (size_t)(ye_lo ? 0 : 1) * wys + ((size_t)s * wNI + gi) * wP + yqe] = sPsi[3][o];
         if constexpr (ADJ) {
-            double* g0 = gshot + ((size_t)2 * s * NI + gi) * NJ + gj;
+            double* g0 = gshot + ((size_t)2 * s * wNI + gi) * wNJ + gj;
             g0[0] = g0v[m] + ae[m];
-            g0[(size_t)NI * NJ] = g1v[m] + as_[m];
+            g0[(size_t)wNI * wNJ] = g1v[m] + as_[m];
         }
     }
 }
