@@ -533,18 +533,23 @@ def uneven_bench(dev):
 def serve_bench(dev, seconds, rate):
     """Serving loop under Poisson arrivals: requests (grid uniform over
     128x128, 256x128, 512x256, 1024x512; 1-32 shots; 1500-5000 steps by grid
-    diagonal) are generated beforehand, submitted from a thread at their
-    arrival times, admitted by device bytes and continuously batched.  Reports
-    p50/p99 latency, sustained throughput and the largest difference between a
-    request served in the mix and the same request served alone (8 sampled)."""
+    diagonal) are submitted from a thread at their arrival times, admitted by
+    device bytes and continuously batched.  Reports p50/p99 latency, sustained
+    throughput and the largest difference between a request served in the mix
+    and the same request served alone (8 sampled).  Up to 200 requests are
+    generated beforehand; longer runs generate each request in the submitting
+    thread just before its arrival (same sequence) and keep the inputs and
+    results of the 8 sampled requests only, so memory stays bounded."""
     import threading
     import numpy as np
     import torch
     from fastgpr.serving import Request, Server, poisson_trace, serving_request
     rng = np.random.default_rng(4)
     arrivals = poisson_trace(rate, seconds, seed=4)
-    reqs = [serving_request(i, rng) for i in range(len(arrivals))]
-    offered = sum(r.wl.cell_updates_per_gradient() for r in reqs)
+    lazy = len(arrivals) > 200
+    sampled = set(k * len(arrivals) // 8 for k in range(8))
+    reqs = [] if lazy else [serving_request(i, rng) for i in range(len(arrivals))]
+    offered_cu = [0]
     # warm-up: one small request of every grid (graph and allocator set-up)
     warm = Server(dev)
     for g in ((128, 128), (256, 128), (512, 256), (1024, 512)):
@@ -559,7 +564,11 @@ def serve_bench(dev, seconds, rate):
     t0 = time.perf_counter()
 
     def submitter():
-        for at, r in zip(arrivals, reqs):
+        for i, at in enumerate(arrivals):
+            if lazy:
+                reqs.append(serving_request(i, rng))
+            r = reqs[i]
+            offered_cu[0] += r.wl.cell_updates_per_gradient()
             d = t0 + at - time.perf_counter()
             if d > 0:
                 time.sleep(d)
@@ -577,9 +586,13 @@ def serve_bench(dev, seconds, rate):
             time.sleep(0.0005)
         now = time.perf_counter()
         if now - last > 20:
-            print(f"serving: {srv.stats.completed}/{len(reqs)} done, {len(srv.active)} active, "
+            print(f"serving: {srv.stats.completed}/{len(arrivals)} done, {len(srv.active)} active, "
                   f"{len(srv.waiting)} waiting", file=sys.stderr, flush=True)
             last = now
+            if lazy:                     # completed requests outside the sample: drop their data
+                for r in reqs[:]:
+                    if r.done_at is not None and r.rid not in sampled and r.result is not None:
+                        r.result, r.obs, r.model = None, None, None
     torch.cuda.synchronize()
     srv.stats.finished = time.perf_counter()
     m1 = time.monotonic_ns()
@@ -587,10 +600,11 @@ def serve_bench(dev, seconds, rate):
     # alone vs in-mix on a sample of the completed requests
     worst = 0.0
     done = [r for r in reqs if r.result and "er" in r.result]
+    picks = [r for r in done if r.rid in sampled] if lazy else done[:: max(1, len(done) // 8)][:8]
     budget = srv.budget
     del srv
     torch.cuda.empty_cache()
-    for r in done[:: max(1, len(done) // 8)][:8]:
+    for r in picks:
         a = Server(dev, hbm_budget=budget)
         ra = a.submit(Request(r.rid, r.wl, r.model, r.obs, tb=r.tb))
         a.run()
@@ -607,7 +621,7 @@ def serve_bench(dev, seconds, rate):
         "sustained_gcell_updates_per_s": round(summ["sustained_gcell_updates_per_s"], 3),
         "completed": summ["completed"], "requests": len(reqs),
         "arrival_window_s": seconds, "rate_per_s": rate, "wall_s": round(summ["wall_s"], 2),
-        "offered_gcell_updates_per_s": round(offered / seconds / 1e9, 3),
+        "offered_gcell_updates_per_s": round(offered_cu[0] / seconds / 1e9, 3),
         "max_concurrent_requests": summ["max_concurrent_requests"],
         "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
         "launches": summ["launches"], "ticks": summ["ticks"],
