@@ -61,6 +61,8 @@ class Stats:
     max_active: int = 0
     ticks: int = 0
     tick_host_s: float = 0.0        # host time spent packing and launching ticks
+    oom_retries: int = 0            # admissions put back because the device ran out of memory
+    refused: int = 0                # requests refused (larger than the budget, or no memory)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -71,6 +73,7 @@ class Stats:
                 "sustained_gcell_updates_per_s": self.cell_updates / wall / 1e9,
                 "wall_s": wall, "launches": self.launches, "ticks": self.ticks,
                 "host_ms_per_tick": 1e3 * self.tick_host_s / max(self.ticks, 1),
+                "oom_retries": self.oom_retries, "refused": self.refused,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
 
 
@@ -94,6 +97,7 @@ class Server:
         self.max_inflight = max_inflight_ticks
         self._tick_events: list = []
         self._lock = threading.Lock()
+        self._hold = None               # completed count at the last out-of-memory admission
 
     # --- client side -----------------------------------------------------------
     def submit(self, req: Request) -> Request:
@@ -122,17 +126,39 @@ class Server:
                 self.waiting.append(self.pending.get_nowait())
             except queue.Empty:
                 break
+        if self._hold is not None:
+            if self.stats.completed == self._hold:
+                return                     # nothing has freed memory since the last failure
+            self._hold = None
         still = []
-        for req in self.waiting:
+        for k, req in enumerate(self.waiting):
             tb, C, nbytes = self._plan(req)
             if nbytes > self.budget:
                 req.result = {"error": f"request needs {nbytes} bytes, budget {self.budget}"}
                 req.done_at = time.perf_counter()
+                self.stats.refused += 1
                 continue
             if self.in_use + nbytes > self.budget:
                 still.append(req)          # FIFO order kept; later small ones may pass
                 continue
-            job = self._make_job(req, tb, C)
+            try:
+                job = self._make_job(req, tb, C)
+            except torch.OutOfMemoryError:
+                # The device holds less than the byte accounting says (memory
+                # cached by torch in pieces too small for this request): release
+                # the unused cache and keep this request and the ones behind it
+                # waiting until a running request has finished.  With nothing
+                # running, nothing will free memory: refuse the request.
+                torch.cuda.empty_cache()
+                if not self.active:
+                    req.result = {"error": "device out of memory with no request running"}
+                    req.done_at = time.perf_counter()
+                    self.stats.refused += 1
+                    continue
+                self.stats.oom_retries += 1
+                self._hold = self.stats.completed
+                still.extend(self.waiting[k:])
+                break
             self.sched.add(job)
             self.active.append((req, job, nbytes))
             self.in_use += nbytes

@@ -71,3 +71,34 @@ def test_forward_requests_and_refusal(hip_device):
     big = tiny.submit(_requests(1, seed=13)[0])
     summary = tiny.run()
     assert "error" in big.result and summary["completed"] == 0
+
+
+def test_out_of_memory_at_admission_is_retried(hip_device, monkeypatch):
+    """A job whose buffers do not fit (the device fuller than the byte
+    accounting says) goes back to waiting and is admitted again once a running
+    request has finished; its result is bitwise the one it gives alone.  The
+    failure is injected on the host: the first construction of request 1's
+    job raises torch.OutOfMemoryError before allocating anything."""
+    from fastgpr.serving import Server
+    dev = hip_device
+    reqs = _requests(3, seed=13)
+    srv = Server(dev, hbm_budget=1 << 36)
+    real = srv._make_job
+    failed = []
+
+    def flaky(req, tb, C):
+        if req.rid == 1 and not failed:
+            failed.append(req.rid)
+            raise torch.OutOfMemoryError("injected")
+        return real(req, tb, C)
+    monkeypatch.setattr(srv, "_make_job", flaky)
+    for r in reqs:
+        srv.submit(r)
+    stats = srv.run()
+    torch.cuda.synchronize()
+    assert failed == [1] and stats["oom_retries"] == 1 and stats["completed"] == 3
+    for r in reqs:
+        assert "er" in r.result, r.result
+        a = _alone(r, dev)
+        for k in ("er", "se", "traces"):
+            assert torch.equal(r.result[k], a.result[k]), (r.rid, k)
