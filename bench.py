@@ -624,7 +624,8 @@ def serve_bench(dev, seconds, rate):
         "offered_gcell_updates_per_s": round(offered_cu[0] / seconds / 1e9, 3),
         "max_concurrent_requests": summ["max_concurrent_requests"],
         "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
-        "refused": summ["refused"], "oom_retries": summ["oom_retries"],
+        "refused": summ["refused"], "cache_releases": summ["cache_releases"],
+        "oom_retries": summ["oom_retries"],
         "launches": summ["launches"], "ticks": summ["ticks"],
         "host_ms_per_tick": round(summ["host_ms_per_tick"], 3),
         "window_ns": [m0, m1],

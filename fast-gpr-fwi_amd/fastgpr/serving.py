@@ -61,6 +61,7 @@ class Stats:
     max_active: int = 0
     ticks: int = 0
     tick_host_s: float = 0.0        # host time spent packing and launching ticks
+    cache_releases: int = 0         # admissions that fitted only after releasing torch's cache
     oom_retries: int = 0            # admissions put back because the device ran out of memory
     refused: int = 0                # requests refused (larger than the budget, or no memory)
 
@@ -73,7 +74,8 @@ class Stats:
                 "sustained_gcell_updates_per_s": self.cell_updates / wall / 1e9,
                 "wall_s": wall, "launches": self.launches, "ticks": self.ticks,
                 "host_ms_per_tick": 1e3 * self.tick_host_s / max(self.ticks, 1),
-                "oom_retries": self.oom_retries, "refused": self.refused,
+                "cache_releases": self.cache_releases, "oom_retries": self.oom_retries,
+                "refused": self.refused,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
 
 
@@ -141,15 +143,22 @@ class Server:
             if self.in_use + nbytes > self.budget:
                 still.append(req)          # FIFO order kept; later small ones may pass
                 continue
-            try:
-                job = self._make_job(req, tb, C)
-            except torch.OutOfMemoryError:
-                # The device holds less than the byte accounting says (memory
-                # cached by torch in pieces too small for this request): release
-                # the unused cache and keep this request and the ones behind it
+            job = None
+            for attempt in range(2):
+                try:
+                    job = self._make_job(req, tb, C)
+                    break
+                except torch.OutOfMemoryError:
+                    # The device holds less than the byte accounting says:
+                    # memory cached by torch in pieces too small for this
+                    # request.  Release the unused cache and try once more.
+                    torch.cuda.empty_cache()
+            if job is not None and attempt == 1:
+                self.stats.cache_releases += 1
+            if job is None:
+                # Still no room: keep this request and the ones behind it
                 # waiting until a running request has finished.  With nothing
                 # running, nothing will free memory: refuse the request.
-                torch.cuda.empty_cache()
                 if not self.active:
                     req.result = {"error": "device out of memory with no request running"}
                     req.done_at = time.perf_counter()

@@ -75,19 +75,22 @@ def test_forward_requests_and_refusal(hip_device):
 
 def test_out_of_memory_at_admission_is_retried(hip_device, monkeypatch):
     """A job whose buffers do not fit (the device fuller than the byte
-    accounting says) goes back to waiting and is admitted again once a running
-    request has finished; its result is bitwise the one it gives alone.  The
-    failure is injected on the host: the first construction of request 1's
-    job raises torch.OutOfMemoryError before allocating anything."""
+    accounting says) is built again after torch's cache is released; if it
+    still does not fit, it goes back to waiting and is admitted once a running
+    request has finished.  Results are bitwise those of each request alone.
+    Failures are injected on the host, before anything is allocated: request
+    1's job fails twice (held), request 2's once (fits after the release)."""
     from fastgpr.serving import Server
     dev = hip_device
     reqs = _requests(3, seed=13)
     srv = Server(dev, hbm_budget=1 << 36)
     real = srv._make_job
+    to_fail = {1: 2, 2: 1}
     failed = []
 
     def flaky(req, tb, C):
-        if req.rid == 1 and not failed:
+        if to_fail.get(req.rid, 0) > 0:
+            to_fail[req.rid] -= 1
             failed.append(req.rid)
             raise torch.OutOfMemoryError("injected")
         return real(req, tb, C)
@@ -96,7 +99,8 @@ def test_out_of_memory_at_admission_is_retried(hip_device, monkeypatch):
         srv.submit(r)
     stats = srv.run()
     torch.cuda.synchronize()
-    assert failed == [1] and stats["oom_retries"] == 1 and stats["completed"] == 3
+    assert failed == [1, 1, 2] and stats["completed"] == 3
+    assert stats["oom_retries"] == 1 and stats["cache_releases"] == 1 and stats["refused"] == 0
     for r in reqs:
         assert "er" in r.result, r.result
         a = _alone(r, dev)
