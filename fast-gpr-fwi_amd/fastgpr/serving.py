@@ -99,7 +99,7 @@ class Server:
         self.max_inflight = max_inflight_ticks
         self._tick_events: list = []
         self._lock = threading.Lock()
-        self._hold = None               # completed count at the last out-of-memory admission
+        self._hold = None               # (completed count, bytes) of the last request that did not fit
 
     # --- client side -----------------------------------------------------------
     def submit(self, req: Request) -> Request:
@@ -128,13 +128,14 @@ class Server:
                 self.waiting.append(self.pending.get_nowait())
             except queue.Empty:
                 break
-        if self._hold is not None:
-            if self.stats.completed == self._hold:
-                return                     # nothing has freed memory since the last failure
-            self._hold = None
+        if self._hold is not None and self.stats.completed != self._hold[0]:
+            self._hold = None              # a request has finished: memory was freed
         still = []
         for k, req in enumerate(self.waiting):
             tb, C, nbytes = self._plan(req)
+            if self._hold is not None and nbytes >= self._hold[1]:
+                still.append(req)          # no larger than what just failed to fit may pass
+                continue
             if nbytes > self.budget:
                 req.result = {"error": f"request needs {nbytes} bytes, budget {self.budget}"}
                 req.done_at = time.perf_counter()
@@ -156,18 +157,19 @@ class Server:
             if job is not None and attempt == 1:
                 self.stats.cache_releases += 1
             if job is None:
-                # Still no room: keep this request and the ones behind it
-                # waiting until a running request has finished.  With nothing
-                # running, nothing will free memory: refuse the request.
+                # Still no room: this request, and any not smaller, wait until
+                # a running request has finished; smaller ones may still be
+                # admitted.  With nothing running, nothing will free memory:
+                # refuse the request.
                 if not self.active:
                     req.result = {"error": "device out of memory with no request running"}
                     req.done_at = time.perf_counter()
                     self.stats.refused += 1
                     continue
                 self.stats.oom_retries += 1
-                self._hold = self.stats.completed
-                still.extend(self.waiting[k:])
-                break
+                self._hold = (self.stats.completed, nbytes)
+                still.append(req)
+                continue
             self.sched.add(job)
             self.active.append((req, job, nbytes))
             self.in_use += nbytes
