@@ -49,13 +49,20 @@ def _save_file():
             json.dump({k: list(v) for k, v in _cache.items()}, f, indent=1)
 
 
-def time_variant(G, er, se, mr, tb, reps: int = 12) -> float:
+def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
     """Seconds per time step (forward + adjoint launch of one block, divided
     by its steps) of one variant, as the sweep runs it: the batch split into
     the same shot groups on their streams, each launching its block ``reps``
     times back to back (fgpr_profile_kernel) after a warm-up; HIP events on
     the calling stream bracket the fan-out and the join.  Buffers are
-    allocated before the timed region."""
+    allocated before the timed region.
+
+    ``reps`` is sized for a stable ranking: on the 400 x 200, 13-shot
+    cross-hole batch, 12 reps put the 6- and 8-step variants of the
+    64-row, 1024-thread region within noise of each other (12.2-12.8 and
+    12.4-13.0 us per step over six trials); 96 reps separate them
+    (11.25-11.56 and 12.27-12.36), as full sweeps do (the 8-step variant's
+    gradient is 9 % slower).  The extra timing costs about 0.3 s per grid."""
     K = int(tb[2])
     nt = 4 * K
     g = dataclasses.replace(G, iterations=nt, srcwaves=G.srcwaves[:, :nt].contiguous(),
