@@ -62,7 +62,8 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
     64-row, 1024-thread region within noise of each other (12.2-12.8 and
     12.4-13.0 us per step over six trials); 96 reps separate them
     (11.25-11.56 and 12.27-12.36), as full sweeps do (the 8-step variant's
-    gradient is 9 % slower).  The extra timing costs about 0.3 s per grid."""
+    gradient is 9 % slower).  ``choose`` scales it down for larger blocks
+    (``timing_reps``)."""
     K = int(tb[2])
     nt = 4 * K
     g = dataclasses.replace(G, iterations=nt, srcwaves=G.srcwaves[:, :nt].contiguous(),
@@ -108,6 +109,18 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
     return total / K
 
 
+def timing_reps(G) -> int:
+    """Block launches timed per variant and direction: 96 for a block of
+    1.04 M cell-shots (400 x 200, 13 shots), fewer in proportion for larger
+    blocks, whose launches are longer than the launch-to-launch jitter that
+    96 reps average out on the small one, and never fewer than 12.  The timed
+    window stays near the small grid's (about 3 ms per variant and
+    direction) instead of growing with the grid: 1600 x 800 with 32 shots
+    takes 12 reps (0.8 s of tuning rather than 6 s)."""
+    cells = max(1, (int(G.NI) + 1) * (int(G.NJ) + 1) * int(G.S))
+    return max(12, min(96, round(96 * 401 * 201 * 13 / cells)))
+
+
 def choose(G, er, se, mr, candidates=None) -> tuple:
     """Fastest time-blocked variant for this grid and batch (cached)."""
     _load_file()
@@ -115,10 +128,11 @@ def choose(G, er, se, mr, candidates=None) -> tuple:
     if key in _cache:
         return _cache[key]
     results = {}
+    reps = timing_reps(G)
     for tb in candidates or CANDIDATES:
         try:
             _native.tb_tiles(G.NI, G.NJ, *tb)
-            results[tb] = time_variant(G, er, se, mr, tb)
+            results[tb] = time_variant(G, er, se, mr, tb, reps=reps)
         except (RuntimeError, ValueError):
             continue
     best = min(results, key=results.get)

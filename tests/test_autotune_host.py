@@ -74,3 +74,11 @@ def test_hardware_queue_limit_caps_the_groups(tuner, monkeypatch):
     monkeypatch.setenv("GPU_MAX_HW_QUEUES", "2")     # one side stream with its own queue
     autotune.choose(grid(shot_streams=3), None, None, None, candidates=CANDIDATES)
     assert list(json.loads(path.read_text())) == ["gfx950-test|400x200|P20|S13|rx80|g1"]
+
+
+def test_timing_reps_shrink_with_the_block_and_stay_bounded():
+    sizes = [(40, 30, 2), (400, 200, 13), (500, 250, 22), (800, 400, 11), (1600, 800, 32), (4000, 4000, 64)]
+    reps = [autotune.timing_reps(SimpleNamespace(NI=a, NJ=b, S=s)) for a, b, s in sizes]
+    assert reps[0] == reps[1] == 96                  # the small grids keep the full count
+    assert all(x >= y for x, y in zip(reps, reps[1:]))
+    assert reps[-1] == 12 and min(reps) == 12
