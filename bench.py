@@ -1,14 +1,19 @@
 """Gradient throughput benchmark (one MI355X).
 
-A step is one full gradient -- forward sweep with snapshot storage, L2
-residual, adjoint sweep with FP64 gradient accumulation and normalisation --
-of the cross-hole batch: 400 x 200 grid, dx 0.025 m, 20-cell CPML, 3000
-steps, Ricker 200 MHz, 13 shots in one launch, 80 receivers; seeded
-synthetic layered model.  Inputs are resident on the device before timing.
+A step is one pass of the hot path over one batch: the gradients of the
+uneven batch -- four seeded models, 64 shots: 300x150 / 2000 steps,
+500x250 / 3000, 800x400 / 4500, 1200x600 / 6000, dx 0.025 m, 20-cell CPML,
+Ricker 200 MHz -- each a forward sweep with snapshot storage, L2 residual,
+adjoint sweep with FP64 gradient accumulation and normalisation, computed
+together by ``fastgpr.uneven.UnevenBatch`` from one captured HIP graph.
+Inputs are resident on the device before timing.
 
-Prints one JSON line (metric, value in Gcell-updates/s, roofline of the
-dominant kernel measured with HIP events on the launch stream, and the CPU
-oracle timed on a bounded sample of the same workload).
+Prints one JSON line: the metric in Gcell-updates/s, the roofline of the
+dominant kernel measured with HIP events on the launch stream, the CPU
+oracle timed on a bounded sample of the same workload, and beside them the
+cross-hole batch (400x200, 13 shots, 3000 steps) through ``compute``, the
+same four models one at a time through ``compute``, the plain
+one-shot-per-launch path and a short run of the serving loop (p50 / p99).
 """
 from __future__ import annotations
 
@@ -28,41 +33,51 @@ METRIC = "gradient Gcell-updates/s (fwd+adjoint) + p50/p99 request latency; HBM 
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_FILE = REPO / "profiles" / "traffic.json"
+
+
 def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2"],
+                    help="c3: the uneven batch (default line); c2: the cross-hole batch alone")
+    ap.add_argument("--quick", action="store_true",
+                    help="the C3 line only (no one-at-a-time, C2, plain-path or serving keys)")
+    ap.add_argument("--c3-variant", default="64,1024,6",
+                    help="time-blocking variant of the uneven batch (rows,threads,steps)")
     ap.add_argument("--exec", dest="exec_mode", default="graph", choices=["graph", "eager"])
     ap.add_argument("--chunk", type=int, default=None,
-                    help="steps per captured graph (default: the library's choice)")
+                    help="C2: steps per captured graph (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-block", default="auto",
-                    help="'auto' (autotuned), 'none' (per-step kernels) or rows,threads,steps")
+                    help="C2: 'auto' (autotuned), 'none' (per-step kernels) or rows,threads,steps")
     ap.add_argument("--plain", action="store_true",
-                    help="also time the plain one-shot-per-launch eager path")
+                    help="with --workload c2: also time the plain one-shot-per-launch eager path")
     ap.add_argument("--cpu-steps", type=int, default=3000,
-                    help="time steps of the CPU oracle sample (forward + adjoint)")
+                    help="C2: time steps of the CPU oracle sample (forward + adjoint)")
+    ap.add_argument("--cpu-steps-c3", type=int, default=240,
+                    help="C3: time steps of the CPU oracle sample on the largest model")
     ap.add_argument("--profile-reps", type=int, default=200)
     ap.add_argument("--shot-streams", type=int, default=3,
-                    help="shot groups swept on separate HIP streams (1 = one batch)")
+                    help="C2: shot groups swept on separate HIP streams (1 = one batch)")
+    ap.add_argument("--serve-check", type=float, default=20.0, metavar="SECONDS",
+                    help="seconds of Poisson arrivals of the serving run in the default line "
+                         "(0 = none)")
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="arrival rate of that serving run (requests/s)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch a 256 MiB copy and the dominant kernels a few times "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
-    ap.add_argument("--uneven", action="store_true",
-                    help="time the uneven batch (4 models, 64 shots) through the multi-problem "
-                         "scheduler against one model at a time; prints its own JSON line")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
-                    help="run the serving loop under Poisson arrivals (20 requests/s) for this "
-                         "many seconds of arrivals; prints its own JSON line")
-    ap.add_argument("--rate", type=float, default=20.0, help="serving arrival rate (requests/s)")
+                    help="only run the serving loop under Poisson arrivals for this many "
+                         "seconds of arrivals; prints its own JSON line")
+    ap.add_argument("--rate", type=float, default=20.0, help="--serve arrival rate (requests/s)")
     ap.add_argument("--kernel-busy", nargs=3, metavar=("DB", "T0_NS", "T1_NS"),
                     help="read a rocprofv3 kernel-trace database and print how much of the "
                          "monotonic window [T0, T1] (the serving line's window_ns) had a kernel "
                          "running, with per-kernel totals; uses no GPU")
-    ap.add_argument("--traffic", default=str(Path(__file__).resolve().parent / "profiles" /
-                                             "traffic_c2_crosshole.json"),
-                    help="measured HBM bytes per launch (from the counter runs), if present")
     return ap.parse_args(argv)
 
 
@@ -139,54 +154,262 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the shot batch runs on one MI355X; multi-GPU sharding is design-only).",
+              "(the batch runs on one MI355X; multi-GPU sharding is design-only).",
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
-
-    import fastgpr
-    from fastgpr import _native
-    from fastgpr import solver as S_
     from fastgpr.workloads import c2_crosshole
 
     dev = torch.device("cuda:0")
-    wl = c2_crosshole()
     if args.pmc_probe:
-        pmc_probe(wl, dev, args)
-        return
-    if args.uneven:
-        uneven_bench(dev)
+        pmc_probe(dev, args)
         return
     if args.serve > 0:
-        serve_bench(dev, args.serve, args.rate)
+        print(json.dumps(serve_bench(dev, args.serve, args.rate)), flush=True)
         return
+    if args.workload == "c2":
+        out = c2_line(dev, args)
+        if args.plain:
+            out["plain_path"] = plain_path(dev, out["value"])
+        print(json.dumps(out), flush=True)
+        return
+
+    out = c3_line(dev, args)
+    if not args.quick:
+        log("C3 one model at a time")
+        out["one_model_at_a_time"] = one_at_a_time(dev)
+        out["one_model_at_a_time"]["batched_speedup"] = round(
+            out["value"] / out["one_model_at_a_time"]["value"], 3)
+        log("C2 cross-hole line")
+        c2 = c2_line(dev, args, extras=False)
+        out["c2_crosshole"] = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline")}
+        log("plain path")
+        out["plain_path"] = plain_path(dev, out["value"])
+        if args.serve_check > 0:
+            log("serving")
+            out["serving"] = serve_bench(dev, args.serve_check, args.serve_rate)
+    print(json.dumps(out), flush=True)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _tensors(wl, dev, model):
+    import numpy as np
+    import torch
     t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
-    er_true, se_true, mr = t(wl.er), t(wl.se), t(wl.mr)
-    er0, se0 = t(wl.er0), t(wl.se0)
-    src = torch.tensor(wl.src)
-    rx = torch.tensor(wl.rx)
+    return t(model[0]), t(model[1]), t(wl.mr)
+
+
+def _compute(wl, dev, er, se, mr, **kw):
+    import torch
+    import fastgpr
+    return fastgpr.compute(dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq],
+                           wl.S, er, se, mr, wl.src.shape[1], wl.rx.shape[1],
+                           time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P, tv=False,
+                           customsource=torch.tensor(wl.src), customreceiver=torch.tensor(wl.rx),
+                           **kw)[0]
+
+
+def c3_problems(dev, tbs):
+    """The uneven batch's problems at the smoothed initial models, with the
+    true models' traces as observed data."""
+    import torch
+    import fastgpr
+    from fastgpr.uneven import Problem
+    from fastgpr.workloads import c3_uneven
+    wls = c3_uneven()
+    probs = []
+    for wl, tb in zip(wls, tbs):
+        with torch.no_grad():
+            obs = _compute(wl, dev, *_tensors(wl, dev, (wl.er, wl.se)), time_block=tb).detach()
+        er, se, mr = _tensors(wl, dev, (wl.er0, wl.se0))
+        er, se = torch.clamp(er, min=1), torch.clamp(se, min=0)
+        G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                                   None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                                   er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
+                                   False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
+        probs.append(Problem(G, er, se, mr, obs, tb))
+    return wls, probs
+
+
+def c3_line(dev, args):
+    """The headline: the uneven batch from one captured graph."""
+    import torch
+    from fastgpr.uneven import UnevenBatch
+    tb = tuple(int(v) for v in args.c3_variant.split(","))
+    log("C3: observed traces and problems")
+    wls, probs = c3_problems(dev, [tb] * 4)
+    t0 = time.perf_counter()
+    ub = UnevenBatch(dev, probs)
+    build_s = time.perf_counter() - t0
+    log(f"C3: built and captured in {build_s:.1f} s: {ub.summary()}")
+    for _ in range(args.warmup):
+        ub.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ub.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for r in res:
+        assert torch.isfinite(r["er"]).all() and torch.isfinite(r["se"]).all()
+    cu = ub.cell_updates()
+    value = cu / (elapsed / args.steps) / 1e9
+    log(f"C3: {value:.1f} Gcell-updates/s")
+    roofline, kernels = multi_roofline(ub, tb, args.profile_reps)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wls[3], args.cpu_steps_c3)
+    summ = ub.summary()
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 gradient accumulation",
+        "data": "synthetic (seeded layered + blob models, inputs resident in HBM)",
+        "config": {"workload": "c3_uneven: 4 models, 64 shots: " +
+                               ", ".join(f"{w.NI}x{w.NJ}/{w.nt} steps/{w.S} shots" for w in wls),
+                   "dx_m": wls[0].dx, "cpml": wls[0].P,
+                   "cell_updates_per_step": cu, "time_block": list(tb),
+                   "exec": "one captured HIP graph per step (all ticks, all waves)",
+                   "waves": summ["waves"], "snapshot_rings": summ["rings"],
+                   "recompute_steps": summ["recompute_steps"],
+                   "ticks": summ["ticks"], "launches": summ["launches"],
+                   "build_and_capture_s": round(build_s, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_us": kernels,
+    }
+
+
+def multi_roofline(ub, tb, reps):
+    """Average launch duration of the multi-problem kernels on the largest
+    model's job (one item per launch, as in its wave), timed with HIP events
+    on the launch stream, against the algorithmic bytes of a block and, when
+    the counter runs measured this exact kernel and workload, the HBM bytes
+    it moved."""
+    import ctypes
+    import torch
+    from fastgpr import _native
+    from fastgpr._native import check, lib
+    k = max(range(len(ub.jobs)), key=lambda i: ub.jobs[i].cell_updates())
+    job = ub.jobs[k]
+    G = job.G
+    name = f"c3_model{k}_{G.NI}x{G.NJ}_{G.S}shots"
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    per = {}
+    K = tb[2]
+    for adj, prob, n in ((False, job.pf_plain, 0), (True, job.pa, G.iterations - 1)):
+        item = _native.TBItem()
+        item.prob = ctypes.pointer(prob)
+        item.n0, item.nsteps, item.parity = n, K, 0
+        nbytes = lib.fgpr_tb_table_bytes(1)
+        host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        total = ctypes.c_int()
+        check(lib.fgpr_tb_pack((_native.TBItem * 1)(item), 1, 1 if adj else 0,
+                               ctypes.c_void_p(host.data_ptr()), nbytes, ctypes.byref(total)))
+        table = host.to(G.device)
+
+        def launch(r):
+            for _ in range(r):
+                check(lib.fgpr_tb_launch_table(ctypes.c_void_p(table.data_ptr()), 1, total.value,
+                                               tb[0], tb[1], tb[2], 1 if adj else 0, sp))
+        launch(4)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        launch(reps)
+        e1.record(st)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        kname = f"k_tb_multi<{tb[0]},{tb[1]},{tb[2]},{'true' if adj else 'false'}>"
+        per[kname] = {"us": us, "bytes": K * step_bytes(G.S, G.NI, G.NJ, G.pmlthick),
+                      "share": us * (G.iterations / K)}
+    best = max(per, key=lambda n: per[n]["share"])
+    d = per[best]
+    gbs = d["bytes"] / d["us"] / 1e3
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "basis": "achieved = algorithmic bytes of the block (40 B per cell-update + 16 B per "
+                     "CPML cell and direction, x steps) / launch time; time blocking keeps the "
+                     "fields on chip, so this is an equivalent rate -- physical is what HBM moved",
+            "kernel": f"{best} on {name} (the largest model, one problem per launch)",
+            "workload_key": f"{name}|{best}",
+            "avg_launch_us": round(d["us"], 2), "bytes_per_launch": int(d["bytes"]),
+            "launches_per_step": int(2 * G.iterations / K)}
+    t = measured_traffic(TRAFFIC_FILE, roof["workload_key"])
+    if t is not None:
+        roof["traffic"] = t
+        roof["physical"] = {"achieved": round(t / d["us"] / 1e3, 1),
+                            "frac": round(t / d["us"] / 1e3 / HBM_PEAK_GBS, 4),
+                            "what": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this "
+                                    "kernel on this workload, copy-calibrated, / launch time"}
+    return roof, {n: round(v["us"], 2) for n, v in per.items()}
+
+
+def one_at_a_time(dev):
+    """The same four gradients one model after another through ``compute``
+    (autotuned variant, captured graphs, shot streams, the history kept as
+    far as it fits)."""
+    import torch
+    from fastgpr.workloads import c3_uneven
+    wls = c3_uneven()
+    obs = []
+    with torch.no_grad():
+        for wl in wls:
+            obs.append(_compute(wl, dev, *_tensors(wl, dev, (wl.er, wl.se)),
+                                time_block="auto").detach())
+
+    def run():
+        t0 = time.perf_counter()
+        for wl, o in zip(wls, obs):
+            er, se, mr = _tensors(wl, dev, (wl.er0, wl.se0))
+            er = torch.clamp(er, min=1).requires_grad_(True)
+            se = torch.clamp(se, min=0).requires_grad_(True)
+            pred = _compute(wl, dev, er, se, mr, time_block="auto", checkpoint="auto")
+            (0.5 * ((pred - o) ** 2).sum()).backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run()
+    el = min(run() for _ in range(2))
+    cu = sum(w.cell_updates_per_gradient() for w in wls)
+    return {"value": round(cu / el / 1e9, 3), "seconds": round(el, 3),
+            "what": "compute() per model: autotuned variant, captured graphs, three shot "
+                    "streams, checkpoint='auto' (best of 2 after a warm-up)"}
+
+
+def c2_line(dev, args, extras=True):
+    """The cross-hole batch through ``compute`` (autotuned, three shot
+    streams, captured graphs)."""
+    import torch
+    import fastgpr
+    from fastgpr.workloads import c2_crosshole
+    wl = c2_crosshole()
+    er_true, se_true, mr = _tensors(wl, dev, (wl.er, wl.se))
+    er0, se0, _ = _tensors(wl, dev, (wl.er0, wl.se0))
     tb = (None if args.time_block == "none" else
           "auto" if args.time_block == "auto" else tuple(int(v) for v in args.time_block.split(",")))
-    kw = dict(time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P, tv=False,
-              customsource=src, customreceiver=rx, exec_mode=args.exec_mode,
-              graph_chunk=args.chunk, time_block=tb, shot_streams=args.shot_streams)
-    common = (dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq], wl.S)
-
+    kw = dict(exec_mode=args.exec_mode, graph_chunk=args.chunk, time_block=tb,
+              shot_streams=args.shot_streams)
     with torch.no_grad():
-        obs, _ = fastgpr.compute(*common, er_true, se_true, mr, 1, wl.rx.shape[1], **kw)
-    obs = obs.detach().clone()
+        obs = _compute(wl, dev, er_true, se_true, mr, **kw).detach().clone()
     er = er0.clone().requires_grad_(True)
     se = se0.clone().requires_grad_(True)
 
     def gradient_step():
         er.grad = None
         se.grad = None
-        pred, _ = fastgpr.compute(*common, er, se, mr, 1, wl.rx.shape[1], **kw)
-        loss = 0.5 * ((pred - obs) ** 2).sum()
-        loss.backward()
-        return loss
+        pred = _compute(wl, dev, er, se, mr, **kw)
+        (0.5 * ((pred - obs) ** 2).sum()).backward()
 
     for _ in range(args.warmup):
         gradient_step()
@@ -199,28 +422,28 @@ def main():
     elapsed = time.perf_counter() - t0
     sweeps = fastgpr.solver.record_sweeps(False)
     assert torch.isfinite(er.grad).all() and torch.isfinite(se.grad).all()
-
     cu = wl.cell_updates_per_gradient()
-    ms = 1e3 * elapsed / args.steps
     value = cu / (elapsed / args.steps) / 1e9
-
-    # ---- dominant-kernel roofline: launch durations with HIP events on the
-    # launch stream, on scratch buffers (same shapes as the timed run)
     if tb == "auto":
         from fastgpr.autotune import _cache, _key
         Gk = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
                                     None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
                                     er.detach(), se.detach(), mr, 1, wl.rx.shape[1], wl.dt,
-                                    'HORIPML', None, None, False, False, None, src, rx,
+                                    'HORIPML', None, None, False, False, None,
+                                    torch.tensor(wl.src), torch.tensor(wl.rx),
                                     time_block="auto", shot_streams=args.shot_streams)
         tb_used = _cache[_key(Gk)]
     else:
         tb_used = tb
+    src, rx = torch.tensor(wl.src), torch.tensor(wl.rx)
     roofline, per_kernel = profile_kernels(wl, er.detach(), se.detach(), mr, src, rx, tb_used,
                                            args.profile_reps)
-    roofline["traffic"] = measured_traffic(args.traffic, roofline["kernel"])
-    # Sweeps of the timed region as run (shot groups on their streams): HIP
-    # events on the calling stream before each fan-out and after its join.
+    t = measured_traffic(TRAFFIC_FILE, f"{wl.name}|{roofline['kernel'].split(' ')[0]}")
+    if t is not None:
+        roofline["traffic"] = t
+        us = roofline["avg_launch_us"]
+        roofline["physical"] = {"achieved": round(t / us / 1e3, 1),
+                                "frac": round(t / us / 1e3 / HBM_PEAK_GBS, 4)}
     if tb_used and sweeps:
         blocks = -(-wl.nt // tb_used[2])
         per = {}
@@ -234,45 +457,25 @@ def main():
             roofline["effective"] = {
                 "what": f"timed-region {kind} sweeps (the dominant kernel's direction): wall "
                         "time per whole-batch block, all shot groups",
-                "groups": max(1, min(args.shot_streams, wl.S, fastgpr.solver.max_shot_groups())),
                 "us_per_block": round(blk, 2), "achieved": round(gbs, 1),
-                "frac": round(gbs / HBM_PEAK_GBS, 4),
-                "forward_us_per_block": round(us.get("forward", 0.0), 2),
-                "adjoint_us_per_block": round(us.get("adjoint", 0.0), 2)}
-
-    # ---- CPU baseline: the numpy FP64 oracle on a bounded sample
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, args.cpu_steps)
-
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
     out = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Gcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 fields, f64 gradient accumulation",
         "data": "synthetic (seeded layered model, inputs resident in HBM)",
         "config": {"workload": wl.name, "grid": [wl.NI, wl.NJ], "dx_m": wl.dx, "cpml": wl.P,
                    "nt": wl.nt, "shots": wl.S, "receivers": int(wl.rx.shape[1]),
                    "cell_updates_per_gradient": cu, "exec": args.exec_mode,
-                   "graph_chunk": (args.chunk if args.chunk else
-                                   fastgpr.solver.TB_GRAPH_LAUNCHES * tb_used[2] if tb_used else 32),
                    "time_block": list(tb_used) if tb_used else None},
         "roofline": roofline,
-        "cpu_baseline": cpu,
         "kernels_us": {k: round(v["us"], 2) for k, v in per_kernel.items()},
-
     }
-    if args.plain:
-        out["plain_path"] = plain_path(wl, er.detach(), se.detach(), mr, obs, src, rx, cu)
-        out["plain_path"]["speedup"] = round(value / out["plain_path"]["value"], 2)
-    print(json.dumps(out), flush=True)
+    if extras and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+    return out
 
 
 KERNEL_NAMES = {
@@ -343,58 +546,67 @@ def profile_kernels(wl, er, se, mr, src, rx, tb, reps):
     return roofline, per
 
 
-def measured_traffic(path, kernel_name):
-    """HBM bytes per launch of ``kernel_name`` from the committed counter
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration), or None when no measurement of this kernel exists."""
+def measured_traffic(path, key):
+    """HBM bytes per launch of the kernel and workload ``key``
+    ("workload|kernel") from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE per dispatch, copy-calibrated), or None when the counter runs
+    did not measure this exact kernel on this exact workload."""
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel_name.split(" ")[0])
+    k = t.get("measured", {}).get(key)
     return None if k is None else k.get("bytes_per_launch")
 
 
-def pmc_probe(wl, dev, args):
+def pmc_probe(dev, args):
     """Program for the FETCH_SIZE / WRITE_SIZE counter runs: a 256 MiB
-    device copy (the calibration: 256 MiB read + 256 MiB written) and the
-    autotuned variant's forward and adjoint kernels, 5 launches each."""
-    import numpy as np
+    device copy (the calibration: 256 MiB read + 256 MiB written), then the
+    headline's multi-problem kernels on the uneven batch's largest model and
+    the cross-hole batch's time-blocked kernels, 5 launches each; prints the
+    order of the dispatches."""
     import torch
-    import fastgpr
-    from fastgpr.autotune import _cache, _key
+    from fastgpr.uneven import UnevenBatch
+    from fastgpr.workloads import c2_crosshole
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    t = lambda x: torch.tensor(np.asarray(x, np.float32))[..., None].to(dev)
-    er, se, mr = t(wl.er0), t(wl.se0), t(wl.mr)
-    src, rx = torch.tensor(wl.src), torch.tensor(wl.rx)
-    tb = (None if args.time_block == "none" else
-          "auto" if args.time_block == "auto" else tuple(int(v) for v in args.time_block.split(",")))
-    if tb == "auto":
-        Gk = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
-                                    None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
-                                    er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
-                                    False, False, None, src, rx)
-        from fastgpr.autotune import choose
-        tb = choose(Gk, er, se, mr)
-    profile_kernels(wl, er, se, mr, src, rx, tb, 5)
+    tb = tuple(int(v) for v in args.c3_variant.split(","))
+    wls, probs = c3_problems(dev, [tb] * 4)
+    ub = UnevenBatch(dev, probs)
+    ub.run()
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "done", "time_block": list(tb) if tb else None}), flush=True)
+    roof, _ = multi_roofline(ub, tb, 5)
+    torch.cuda.synchronize()
+    del ub, probs
+    torch.cuda.empty_cache()
+    wl = c2_crosshole()
+    er, se, mr = _tensors(wl, dev, (wl.er0, wl.se0))
+    profile_kernels(wl, er, se, mr, torch.tensor(wl.src), torch.tensor(wl.rx), (64, 1024, 6), 5)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "c3": roof["workload_key"],
+                      "c2": f"{wl.name}|k_tb<64,1024,6,true>"}), flush=True)
 
 
-def plain_path(wl, er, se, mr, obs, src, rx, cu):
-    """The plain GPU path: one shot per launch sequence, per-step kernels
-    launched eagerly from the host loop, no graph, no batching, no time
-    blocking.  One full gradient of the same workload, shot by shot."""
+def plain_path(dev, value):
+    """The plain GPU path on the cross-hole batch: one shot per launch
+    sequence, per-step kernels launched eagerly from the host loop, no
+    graph, no batching, no time blocking; one full gradient, shot by shot.
+    ``value`` (the headline) over its rate is the speedup of the batched
+    path."""
     import torch
     import fastgpr
-    dev = er.device
-    er = er.clone().requires_grad_(True)
-    se = se.clone().requires_grad_(True)
+    from fastgpr.workloads import c2_crosshole
+    wl = c2_crosshole()
+    er, se, mr = _tensors(wl, dev, (wl.er0, wl.se0))
+    with torch.no_grad():
+        obs = _compute(wl, dev, *_tensors(wl, dev, (wl.er, wl.se)), time_block="auto")
+    er = er.requires_grad_(True)
+    se = se.requires_grad_(True)
+    src, rx = torch.tensor(wl.src), torch.tensor(wl.rx)
     common = (dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq], 1)
 
     def one_gradient():
@@ -411,123 +623,10 @@ def plain_path(wl, er, se, mr, obs, src, rx, cu):
     one_gradient()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return {"value": round(cu / el / 1e9, 3), "unit": "Gcell-updates/s",
-            "ms_per_gradient": round(el * 1e3, 1),
+    rate = wl.cell_updates_per_gradient() / el / 1e9
+    return {"value": round(rate, 3), "unit": "Gcell-updates/s", "workload": wl.name,
+            "ms_per_gradient": round(el * 1e3, 1), "speedup": round(value / rate, 2),
             "what": "per-step eager launches, one shot per launch sequence, no graph"}
-
-
-def _job_for(wl, dev, tb, checkpoint):
-    """A gradient job of workload ``wl`` (initial model, zero observed data),
-    checkpointed when ``checkpoint`` is true."""
-    import math
-    import numpy as np
-    import torch
-    import fastgpr
-    from fastgpr.batching import Job
-    t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
-    er, se, mr = torch.clamp(t(wl.er0), min=1), torch.clamp(t(wl.se0), min=0), t(wl.mr)
-    G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
-                               None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
-                               er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
-                               False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
-    C = None
-    if checkpoint:
-        K = tb[2]
-        c = int(math.ceil(math.sqrt(3.0 * wl.nt)))
-        C = ((c + 2 * K - 1) // (2 * K)) * 2 * K
-    obs = torch.zeros((wl.S, wl.nt, wl.rx.shape[1]), dtype=torch.float32, device=dev)
-    return Job(G, er, se, mr, obs=obs, tb=tb, checkpoint=C), C
-
-
-def uneven_bench(dev):
-    """Uneven batch: 4 models of different sizes and step counts, 64 shots,
-    advanced together by multi-problem launches (one per direction and
-    variant per tick), each model with the time-blocking variant the
-    autotuner picks for its grid among those the multi-problem kernel has,
-    against the same gradients run one model at a time through ``compute``
-    (autotuned time-blocked kernels, captured graphs, shot streams).
-    Checkpoint recompute is not counted."""
-    import numpy as np
-    import torch
-    import fastgpr
-    from fastgpr import autotune
-    from fastgpr.batching import MULTI_VARIANTS, Scheduler, job_bytes
-    from fastgpr.workloads import c3_uneven
-    wls = c3_uneven()
-    # Per grid, the variant with the best throughput when the device is full
-    # (in a multi-problem launch other jobs fill a job's launch tail): tuned
-    # on the grid's acquisition repeated to at least 64 shots.
-    tbs = []
-    for w in wls:
-        t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
-        er, se, mr = torch.clamp(t(w.er0), min=1), torch.clamp(t(w.se0), min=0), t(w.mr)
-        rep = -(-64 // w.S)
-        G = fastgpr.initialization(dev, w.dx, w.NJ, w.NI, 1, w.time_window, ['hertzian', 'z'],
-                                   None, None, None, None, w.P, ['ricker', 1, w.freq], w.S * rep,
-                                   er, se, mr, 1, w.rx.shape[1], w.dt, 'HORIPML', None, None,
-                                   False, False, None, torch.tensor(np.tile(w.src, (rep, 1, 1))),
-                                   torch.tensor(np.tile(w.rx, (rep, 1, 1))), shot_streams=1)
-        tbs.append(autotune.choose(G, er, se, mr, candidates=MULTI_VARIANTS))
-    torch.cuda.empty_cache()
-    free, _ = torch.cuda.mem_get_info(dev)
-    # Checkpoint until every job fits in 80 % of the free memory, choosing
-    # first the jobs whose recompute (one more forward sweep) costs the
-    # fewest cell updates per byte the checkpointing frees.
-    full = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, w.nt, tb=tb)
-            for w, tb in zip(wls, tbs)]
-    small = [job_bytes(w.NI, w.NJ, w.S, w.nt, w.rx.shape[1], w.P, 144, tb=tb)
-             for w, tb in zip(wls, tbs)]
-    ck = [False] * len(wls)
-    order = sorted(range(len(wls)),
-                   key=lambda k: wls[k].S * wls[k].NI * wls[k].NJ * wls[k].nt
-                   / max(full[k] - small[k], 1))
-    for k in order:
-        if sum(small[i] if ck[i] else full[i] for i in range(len(wls))) <= 0.8 * free:
-            break
-        ck[k] = True
-    cu = sum(w.cell_updates_per_gradient() for w in wls)
-
-    def batched():
-        jobs = [_job_for(w, dev, tb, c) for w, tb, c in zip(wls, tbs, ck)]
-        sch = Scheduler(dev)
-        for j, _ in jobs:
-            sch.add(j)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sch.run()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0, sch.launches, [c for _, c in jobs]
-
-    batched()                                    # warm-up (graphs, allocator)
-    el_b, launches, ckpts = batched()
-
-    def one_at_a_time():
-        t0 = time.perf_counter()
-        for w in wls:
-            t = lambda a: torch.tensor(np.asarray(a, np.float32))[..., None].to(dev)
-            er = torch.clamp(t(w.er0), min=1).requires_grad_(True)
-            se = torch.clamp(t(w.se0), min=0).requires_grad_(True)
-            pred, _ = fastgpr.compute(dev, w.dx, w.NJ, w.NI, 1, ['hertzian', 'z'],
-                                      ['ricker', 1, w.freq], w.S, er, se, t(w.mr), 1,
-                                      w.rx.shape[1], time_windows=w.time_window, dt=w.dt,
-                                      pmlthick=w.P, tv=False, customsource=torch.tensor(w.src),
-                                      customreceiver=torch.tensor(w.rx), time_block="auto",
-                                      checkpoint="auto")
-            (0.5 * (pred ** 2).sum()).backward()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    one_at_a_time()
-    el_s = one_at_a_time()
-    print(json.dumps({
-        "metric": "uneven batch gradient Gcell-updates/s (fwd+adjoint)",
-        "value": round(cu / el_b / 1e9, 3), "unit": "Gcell-updates/s",
-        "seconds": round(el_b, 3), "launches": launches,
-        "one_model_at_a_time": {"value": round(cu / el_s / 1e9, 3), "seconds": round(el_s, 3)},
-        "speedup": round(el_s / el_b, 3),
-        "config": {"workload": "4 models, 64 shots: " + ", ".join(w.name + f"/{w.nt}" for w in wls),
-                   "cell_updates": cu, "time_block": [list(tb) for tb in tbs],
-                   "checkpoint_interval": ckpts}}), flush=True)
 
 
 def serve_bench(dev, seconds, rate):
@@ -586,7 +685,7 @@ def serve_bench(dev, seconds, rate):
             time.sleep(0.0005)
         now = time.perf_counter()
         if now - last > 20:
-            print(f"serving: {srv.stats.completed}/{len(arrivals)} done, {len(srv.active)} active, "
+            print(f"[bench] serving: {srv.stats.completed}/{len(arrivals)} done, {len(srv.active)} active, "
                   f"{len(srv.waiting)} waiting", file=sys.stderr, flush=True)
             last = now
             if lazy:                     # completed requests outside the sample: drop their data
@@ -604,17 +703,19 @@ def serve_bench(dev, seconds, rate):
     budget = srv.budget
     del srv
     torch.cuda.empty_cache()
+    picks = picks if picks else []
     for r in picks:
         a = Server(dev, hbm_budget=budget)
         ra = a.submit(Request(r.rid, r.wl, r.model, r.obs, tb=r.tb))
         a.run()
         torch.cuda.synchronize()
+        del a
         if "er" not in ra.result:
             raise RuntimeError(f"request {r.rid} served alone failed: {ra.result}")
         for k in ("er", "se", "traces"):
             x, y = r.result[k].double(), ra.result[k].double()
             worst = max(worst, float((x - y).abs().max() / max(y.abs().max(), 1e-30)))
-    print(json.dumps({
+    return {
         "metric": "serving: p50/p99 request latency, sustained gradient Gcell-updates/s",
         "p50_latency_s": round(summ["p50_latency_s"], 3),
         "p99_latency_s": round(summ["p99_latency_s"], 3),
@@ -624,12 +725,13 @@ def serve_bench(dev, seconds, rate):
         "offered_gcell_updates_per_s": round(offered_cu[0] / seconds / 1e9, 3),
         "max_concurrent_requests": summ["max_concurrent_requests"],
         "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
-        "refused": summ["refused"], "cache_releases": summ["cache_releases"],
-        "oom_retries": summ["oom_retries"],
+        "refused": summ["refused"], "holds": summ["holds"],
+        "partial_histories": summ["partial_histories"],
+        "recompute_share": round(summ["recompute_share"], 4),
         "launches": summ["launches"], "ticks": summ["ticks"],
         "host_ms_per_tick": round(summ["host_ms_per_tick"], 3),
         "window_ns": [m0, m1],
-        "alone_vs_mix_max_rel_diff": worst}), flush=True)
+        "alone_vs_mix_max_rel_diff": worst}
 
 
 def cpu_baseline(wl, nsteps: int):

@@ -602,6 +602,24 @@ constexpr int kTBCols = 64;
 constexpr int kPmax = 128;
 constexpr int kWaveList = 128;   // receiver entries one wave can hold
 
+// Snapshot block of one (slot, shot, tile) in the time-blocked layout of
+// variant (RI, NT, K): waves kWaveLo .. kWaveHi are those that own a
+// tile-centre row (rows K .. K + TI - 1 of the region), each storing its CPT
+// rows for the TJ centre lanes.
+template <int RI, int NT, int K>
+struct TBSnapGeom {
+    static constexpr int W = NT / 64, CPT = RI / W;
+    static constexpr int TI = RI - 2 * K - 1, TJ = kTBCols - 2 * K - 1;
+    static constexpr int kWaveLo = K / CPT, kWaveHi = (K + TI - 1) / CPT;
+    static constexpr int kBlock = (kWaveHi - kWaveLo + 1) * TJ * CPT;
+};
+
+// The same block size for a run-time variant.
+__host__ __device__ inline int64_t tb_snap_block(int ri, int nt, int k) {
+    const int cpt = ri / (nt / 64), TI = ri - 2 * k - 1, TJ = kTBCols - 2 * k - 1;
+    return (int64_t)((k + TI - 1) / cpt - k / cpt + 1) * TJ * cpt;
+}
+
 // Exact unsigned division by a launch-constant d through a multiply and a
 // shift (Granlund and Montgomery, "Division by invariant integers using
 // multiplication", PLDI 1994, round-up variant with the add): for
@@ -641,8 +659,12 @@ __device__ __forceinline__ T ld_arg(const T* field) {
     return *(const __attribute__((address_space(4))) T*)field;
 }
 
-__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+__host__ __device__ __forceinline__ int fdiv(int n, FastDiv f) {
+#if defined(__HIP_DEVICE_COMPILE__)
     const unsigned t = __umulhi((unsigned)n, f.m);
+#else
+    const unsigned t = (unsigned)(((uint64_t)(unsigned)n * f.m) >> 32);   // host copy (tests)
+#endif
     return (int)(((uint64_t)t + (unsigned)n) >> f.l);
 }
 
@@ -670,9 +692,12 @@ struct TBArgs {
 // general (CPML / edge) items, which cost more (their waves run the CPML and
 // edge copies of the step loop): the cheap ones finish early and free their
 // CUs for the expensive ones, instead of an expensive tail.  An item is (tile, shot), shots innermost, so the shots of
-// a tile (which share coefficients) meet in one XCD's L2.  Returns the
-// shot and the tile of block `wg` (a bijection over [0, nwg)).
-__device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg, int& s, int& tile) {
+// a tile (which share coefficients) meet in one XCD's L2.  Sets the shot
+// and the tile of block `wg` (a bijection over [0, nwg)) and returns whether
+// the item is one of the interior ones.  Compiled for the host too
+// (fgpr_tb_map), so its arithmetic can be checked without a GPU.
+__host__ __device__ __forceinline__ bool tb_item(const TBArgs& a, int S, int wg, int nwg, int& s,
+                                                 int& tile) {
     const int tjc = a.tj_count;
     const int ni = a.int_ni, nj = a.int_nj;
     const int Ni = ni * nj * S;                      // interior items
@@ -688,22 +713,23 @@ __device__ __forceinline__ void tb_item(const TBArgs& a, int S, int wg, int nwg,
         const int q2 = fdiv(kt, a.fd_nj);
         const int ti = a.int_ti0 + q2;
         tile = ti * tjc + a.int_tj0 + (kt - q2 * nj);
-        return;
+        return true;
     }
     const int g = (base - ia) + (t - icnt), gt = fdiv(g, a.fd_S);
     s = g - gt * S;
     // general tiles in row-major order with the interior rectangle skipped
     const int before = a.int_ti0 * tjc;              // rows above the rectangle
     const int per = tjc - nj;                        // general tiles of a rectangle row
-    if (ni == 0 || gt < before) { tile = gt; return; }
+    if (ni == 0 || gt < before) { tile = gt; return false; }
     const int u = gt - before;
     if (u < ni * per) {
         const int q3 = fdiv(u, a.fd_per);
         const int ti = a.int_ti0 + q3, c = u - q3 * per;
         tile = ti * tjc + (c < a.int_tj0 ? c : c + nj);
-        return;
+        return false;
     }
     tile = (a.int_ti0 + ni) * tjc + (u - ni * per);
+    return false;
 }
 
 // Load through the scalar cache (a wave-uniform address of read-only data).
@@ -1024,16 +1050,20 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     auto snap_ref = [&](int sl, int m) -> float& {
         return (snap_wave + (int64_t)sl * snap_slot + m * pitch)[gj];
     };
-    // Time-blocked layout: per (slot, shot, tile) a block of RI x 64 floats
-    // ordered [wave][lane][row], so each thread's CPT rows are contiguous
-    // (one vector store / load per step) and each block is written by one
-    // workgroup.  Only tile-centre cells are ever read back.
-    constexpr int kBlk = RI * 64;
+    // Time-blocked layout: per (slot, shot, tile) a block holding the waves
+    // that own tile-centre rows and the tile-centre lanes only, ordered
+    // [wave][centre lane][row], so each thread's CPT rows are contiguous (one
+    // vector store / load per step) and each block is written by one
+    // workgroup.  Halo lanes and halo-only waves take no memory.
+    constexpr int kBlk = TBSnapGeom<RI, NT, K>::kBlock;
+    constexpr int kWlo = TBSnapGeom<RI, NT, K>::kWaveLo;
     const bool tiled = d.snap_tiled != 0;
     const int64_t tslot = (int64_t)d.S * ntiles * kBlk;
-    float* const tsnap = d.snaps ? d.snaps + ((int64_t)s * ntiles + tile) * kBlk + (w * 64 + lane) * CPT
-                                 : nullptr;
     const bool wave_centre = r0 + CPT > K && r0 < K + TI;          // wave-uniform
+    float* const tsnap = (d.snaps && wave_centre && centre_col)
+                             ? d.snaps + ((int64_t)s * ntiles + tile) * kBlk +
+                                   ((w - kWlo) * TJ + (lane - K)) * CPT
+                             : nullptr;
     auto tstore = [&](int sl) {
         float* q = tsnap + sl * tslot;
 #pragma unroll
@@ -1311,17 +1341,19 @@ k_tb_multi(const unsigned char* __restrict__ table, int nitems) {
 // slot from a saved state.  Cells outside the domain are written as zero.
 __global__ void __launch_bounds__(256) k_snap_pack(Dev d, int RI, int NT, int K, int ti_count,
                                                    int tj_count, int slot, const float* __restrict__ Ez) {
-    const int ntiles = ti_count * tj_count, blk = RI * 64, cpt = RI / (NT / 64);
+    const int ntiles = ti_count * tj_count, cpt = RI / (NT / 64);
+    const int TJ = kTBCols - 2 * K - 1, wlo = K / cpt;
+    const int blk = (int)tb_snap_block(RI, NT, K);
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= (int64_t)d.S * ntiles * blk) return;
     const int64_t b = e / blk;
     const int rem = (int)(e - b * blk);
     const int wl = rem / cpt, m = rem - wl * cpt;
-    const int w = wl >> 6, lane = wl & 63;
+    const int w = wlo + wl / TJ, lane = K + wl % TJ;
     const int sh = (int)(b / ntiles), tile = (int)(b - (int64_t)sh * ntiles);
     const int ti = tile / tj_count, tj = tile - ti * tj_count;
     const int gi = ti * (RI - 2 * K - 1) - K + w * cpt + m;
-    const int gj = tj * (kTBCols - 2 * K - 1) - K + lane;
+    const int gj = tj * TJ - K + lane;
     const bool in = gi >= 0 && gi < d.NI && gj >= 0 && gj < d.NJ;
     d.snaps[(int64_t)slot * d.S * ntiles * blk + e] =
         in ? Ez[((int64_t)sh * (d.NI + 1) + gi) * d.pitch + gj] : 0.f;
@@ -2139,10 +2171,44 @@ int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_ite
     return FGPR_OK;
 }
 
+int64_t fgpr_tb_map(const fgpr_problem* p, int adj, int32_t* shot, int32_t* tile, int8_t* interior,
+                    int64_t n) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if ((rc = tb_check(*p, adj != 0))) return rc;
+    const int64_t nwg = (int64_t)p->tb_ti * p->tb_tj * p->S;
+    if (!shot || !tile || !interior || n < nwg) return fail(FGPR_EINVAL, "output arrays shorter than the launch");
+    const TBArgs a = make_tb(*p, to_dev(*p), 0, adj != 0);
+    for (int64_t wg = 0; wg < nwg; ++wg) {
+        int s_ = 0, t_ = 0;
+        interior[wg] = tb_item(a, p->S, (int)wg, (int)nwg, s_, t_) ? 1 : 0;
+        shot[wg] = s_;
+        tile[wg] = t_;
+    }
+    return nwg;
+}
+
+int64_t fgpr_fastdiv_mismatches(int d, int64_t n0, int64_t n1) {
+    if (d < 1 || n0 < 0 || n1 > ((int64_t)1 << 31) || n0 > n1) return -1;
+    const FastDiv f = make_fastdiv(d);
+    int64_t bad = 0;
+    for (int64_t n = n0; n < n1; ++n)
+        bad += fdiv((int)n, f) != (int)(n / d);
+    return bad;
+}
+
 int64_t fgpr_snap_slot_floats(const fgpr_problem* p) {
     if (!p) return -1;
     if (!p->snap_tiled) return (int64_t)p->S * p->NI * p->pitch;
-    return (int64_t)p->S * p->tb_ti * p->tb_tj * p->tb_ri * 64;
+    if (!tb_kernel(p->tb_ri, p->tb_nt, p->tb_k, false)) return -1;
+    return (int64_t)p->S * p->tb_ti * p->tb_tj * tb_snap_block(p->tb_ri, p->tb_nt, p->tb_k);
+}
+
+int64_t fgpr_tb_snap_floats(int NI, int NJ, int tb_ri, int tb_nt, int tb_k) {
+    if (!tb_kernel(tb_ri, tb_nt, tb_k, false) || NI < 1 || NJ < 1) return -1;
+    const int TI = tb_ri - 2 * tb_k - 1, TJ = kTBCols - 2 * tb_k - 1;
+    const int64_t tiles = (int64_t)((NI + 1 + TI - 1) / TI) * ((NJ + 1 + TJ - 1) / TJ);
+    return tiles * tb_snap_block(tb_ri, tb_nt, tb_k);
 }
 
 int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* stream) {
@@ -2151,7 +2217,7 @@ int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* strea
     if (!p->snap_tiled || !p->snaps || !Ez) return fail(FGPR_EINVAL, "snap_pack needs a tiled snapshot buffer and Ez");
     if ((rc = tb_check(*p, false))) return rc;
     if (slot < 0 || slot >= p->nsnap) return fail(FGPR_EINVAL, "snapshot slot outside the buffer");
-    const int64_t n = (int64_t)p->S * p->tb_ti * p->tb_tj * p->tb_ri * 64;
+    const int64_t n = (int64_t)p->S * p->tb_ti * p->tb_tj * tb_snap_block(p->tb_ri, p->tb_nt, p->tb_k);
     hipLaunchKernelGGL(k_snap_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        to_dev(*p), p->tb_ri, p->tb_nt, p->tb_k, p->tb_ti, p->tb_tj, slot, Ez);
     HIP_TRY(hipGetLastError());

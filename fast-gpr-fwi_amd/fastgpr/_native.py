@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
@@ -97,6 +97,10 @@ SIGNATURES = {
     "fgpr_row_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "fgpr_snap_slot_floats": (ctypes.c_int64, [ctypes.POINTER(Problem)]),
+    "fgpr_tb_snap_floats": (ctypes.c_int64, [ctypes.c_int] * 5),
+    "fgpr_tb_map": (ctypes.c_int64, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "fgpr_fastdiv_mismatches": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64]),
     "fgpr_snap_pack": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p]),
     "fgpr_fir": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -149,3 +153,12 @@ def pitch(NJ: int) -> int:
 
 def psi_floats(NI: int, NJ: int, P: int, S: int) -> int:
     return lib.fgpr_psi_floats(NI, NJ, P, S)
+
+
+def tb_snap_floats(NI: int, NJ: int, tb) -> int:
+    """Floats per shot of one snapshot slot in the time-blocked layout of
+    variant ``tb`` = (region rows, threads, steps per launch)."""
+    n = lib.fgpr_tb_snap_floats(NI, NJ, *tb)
+    if n < 0:
+        raise ValueError(f"time-blocking variant {tuple(tb)} is not compiled in")
+    return int(n)

@@ -1,15 +1,24 @@
-"""Checkpoint / recompute gradients for wavefield histories that do not fit.
+"""Gradients whose wavefield history does not fit: keep what fits, recompute
+the rest.
 
 Storing Ez after every step costs nt * S * NI * NJ * 4 bytes (64 GB per shot
-for a 2000 x 1000 grid over 8000 steps).  With an interval C the forward
-sweep keeps only the full solver state (Ez, Hx, Hy and the CPML memory
-variables of every shot) at steps C, 2C, ...  The adjoint then walks the
-segments backwards: restore the state at the segment start, recompute that
-segment's snapshots into a ring of C + 1 slots (slot = step mod (C + 1), so
-one captured graph serves every segment), and run the adjoint over it.
-Memory: about nt / C states plus C + 1 snapshot slots; cost: one extra
-forward sweep.  The recomputed snapshots are bitwise those of the first
-sweep, so the gradient equals the full-storage one exactly.
+for a 2000 x 1000 grid over 8000 steps).  A ``History`` keeps the snapshots
+in a ring of R slots (step n in slot n mod R):
+
+* the forward sweep writes its last R snapshots into the ring and, before
+  that, saves the full solver state (Ez, Hx, Hy and the CPML memory of every
+  shot) at the start of each recompute segment (every C = R - 1 steps);
+* the adjoint first walks the stored tail, steps nt-1 down to h = nt - R + 1,
+  straight from the ring;
+* then it walks the head segments [s0, e) backwards: restore the state at
+  s0, recompute the segment's snapshots into the ring (the slot before s0 is
+  seeded from the saved Ez), run the adjoint over it.
+
+Only the h head steps are computed twice, so the recompute shrinks as the
+ring grows: with R >= nt the history is stored whole (the reference's
+behaviour) and nothing is recomputed.  The recomputed snapshots are bitwise
+those of the first sweep, so every split gives the full-storage gradient
+exactly.
 
 The reference stores the whole history (compute.py:551-553, 710-712); its
 ``nseg`` segmenting is unfinished (compute.py:543-553) and raises when used.
@@ -17,10 +26,8 @@ The reference stores the whole history (compute.py:551-553, 710-712); its
 from __future__ import annotations
 
 import ctypes
-import math
 from dataclasses import dataclass
 
-import numpy as np
 import torch
 
 from . import _native
@@ -28,112 +35,232 @@ from . import solver as _s
 from ._native import check, lib
 
 
-def interval(G, checkpoint) -> int | None:
-    """Steps between saved states, or None for full snapshot storage.
+@dataclass(frozen=True)
+class History:
+    """Snapshot ring of R slots for an nt-step gradient whose adjoint runs
+    down to step ``lowest`` (1 for the reference gradient, 0 for the exact
+    one)."""
+    nt: int
+    R: int
+    lowest: int = 1
 
-    "auto" checkpoints only when the full history would not fit in the free
-    device memory (counting what torch's caching allocator holds unused) less
-    a reserve for the rest of the gradient (5 % of the device, at least
-    8 GiB), with C about sqrt(3 nt) rounded up to a whole number of
-    captured-graph chunks.  A history under a quarter of the device's memory
-    is stored without asking the device for its free memory."""
+    def __post_init__(self):
+        if self.R < 2 or self.nt < 1:
+            raise ValueError(f"snapshot ring of {self.R} slots for {self.nt} steps")
+
+    @property
+    def C(self) -> int:
+        """Steps per recompute segment."""
+        return self.R - 1
+
+    @property
+    def h(self) -> int:
+        """First step of the stored tail (adjoint steps h .. nt-1 read the
+        ring as the forward left it)."""
+        return max(self.nt - self.R + 1, 0)
+
+    def tail(self):
+        """(n_hi, n_lo) of the adjoint over the stored tail, or None."""
+        lo = max(self.h, self.lowest)
+        return (self.nt - 1, lo) if self.nt - 1 >= lo else None
+
+    def segments(self) -> list:
+        """Head segments [s0, e) in forward order whose adjoint range
+        [max(s0, lowest), e - 1] is not empty."""
+        out = []
+        for s0 in range(0, self.h, self.C):
+            e = min(s0 + self.C, self.h)
+            if e - 1 >= max(s0, self.lowest):
+                out.append((s0, e))
+        return out
+
+    def saves(self) -> list:
+        """Steps at which the forward saves the solver state (the starts of
+        the head segments after the first)."""
+        return [s0 for s0, _ in self.segments() if s0 > 0]
+
+    @property
+    def full(self) -> bool:
+        return not self.segments()
+
+    def first_stored(self) -> int:
+        """First step whose snapshot the forward must write to the ring."""
+        return max(self.nt - self.R, 0)
+
+    def recompute_steps(self) -> int:
+        return sum(e - s0 for s0, e in self.segments())
+
+    def nbytes(self, slot_bytes: int, state_bytes: int) -> int:
+        return self.R * slot_bytes + len(self.saves()) * state_bytes
+
+
+def plan(nt: int, slot_bytes: int, state_bytes: int, budget: int, lowest: int = 1) -> History:
+    """The largest ring whose slots and saved states fit in ``budget`` bytes
+    (the whole history when it fits).  Raises MemoryError when even the
+    smallest footprint does not fit."""
+    if nt * slot_bytes <= budget:
+        return History(nt, nt, lowest)
+    R = min(nt, int(budget // max(slot_bytes, 1)))
+    best = None
+    while R >= 2:
+        hist = History(nt, R, lowest)
+        over = hist.nbytes(slot_bytes, state_bytes) - budget
+        if over <= 0:
+            return hist
+        best = hist
+        R -= max(1, -(-over // max(slot_bytes, 1)))
+    need = best.nbytes(slot_bytes, state_bytes) if best else 2 * slot_bytes
+    raise MemoryError(f"snapshot history needs at least ~{need} bytes of device memory, "
+                      f"{budget} available")
+
+
+def slot_bytes(G, tb=None) -> int:
+    """Bytes of one snapshot slot of G's batch (time-blocked layout when the
+    time-blocked reference adjoint reads it)."""
+    tb = G.time_block if tb is None else tb
+    if tb and tb != "auto" and G.options.get("gradient", "reference") != "exact":
+        return 4 * G.S * _native.tb_snap_floats(G.NI, G.NJ, tb)
+    return 4 * G.S * G.NI * _native.pitch(G.NJ)
+
+
+def state_bytes(G) -> int:
+    """Bytes of one saved solver state (fields and CPML memory)."""
+    pitch = _native.pitch(G.NJ)
+    return 4 * (3 * G.S * (G.NI + 1) * pitch + max(_native.psi_floats(G.NI, G.NJ, G.pmlthick, G.S), 1))
+
+
+def history(G, checkpoint) -> History | None:
+    """How the snapshots of G's gradient are kept; None = every snapshot in
+    one buffer (the reference's full history).
+
+    ``checkpoint`` = None / False: full history.  An int C: a ring of C + 1
+    slots (recompute segments of C steps).  "auto": the full history when it
+    fits in the free device memory (counting what torch's caching allocator
+    holds unused) less a reserve for the rest of the gradient (5 % of the
+    device, at least 8 GiB); otherwise the largest ring that does.  A history
+    under a quarter of the device's memory is stored without asking the
+    device for its free memory."""
     if checkpoint is None or checkpoint is False:
         return None
     nt = G.iterations
+    lowest = 0 if G.options.get("gradient", "reference") == "exact" else 1
     if checkpoint == "auto":
-        slot = G.S * G.NI * _native.pitch(G.NJ)
-        if G.time_block and G.options.get("gradient", "reference") != "exact":
-            _, _, ti, tj = _native.tb_tiles(G.NI, G.NJ, *G.time_block)
-            slot = G.S * ti * tj * G.time_block[0] * 64          # time-blocked layout
-        full = nt * slot * 4
+        sb = slot_bytes(G)
+        full = nt * sb
         total = torch.cuda.get_device_properties(G.device).total_memory
         if full <= 0.25 * total:
             return None
         free, _ = torch.cuda.mem_get_info(G.device)
         free += torch.cuda.memory_reserved(G.device) - torch.cuda.memory_allocated(G.device)
-        if full <= free - max(8 << 30, 0.05 * total):
+        budget = int(free - max(8 << 30, 0.05 * total))
+        if full <= budget:
             return None
-        c = int(math.ceil(math.sqrt(3.0 * nt)))
-        q = max(G.graph_chunk, 1)
-        return max(q, ((c + q - 1) // q) * q)
+        return plan(nt, sb, state_bytes(G), budget, lowest)
     c = int(checkpoint)
     if c < 1:
         raise ValueError("checkpoint interval must be >= 1")
-    return c
+    return History(nt, min(c + 1, nt + 1), lowest)
 
 
 class States:
-    """Solver states at steps k*C (k = 1 .. nseg-1), on the device."""
+    """Solver states saved at the given steps, on the device."""
 
-    def __init__(self, b: "_s.Batch", C: int):
-        self.C = C
-        self.nseg = (b.nt + C - 1) // C
-        n = max(self.nseg - 1, 0)
-        self.fields = torch.empty((n,) + tuple(b.fields.shape), dtype=torch.float32,
-                                  device=b.fields.device)
-        self.psi = torch.empty((n,) + tuple(b.psi.shape), dtype=torch.float32, device=b.psi.device)
+    def __init__(self, b: "_s.Batch", steps, alloc=None):
+        self.steps = list(steps)
+        n = len(self.steps)
+        if alloc is None:
+            alloc = lambda shape, dt: torch.empty(shape, dtype=dt, device=b.fields.device)
+        self.fields = alloc((n,) + tuple(b.fields.shape), torch.float32)
+        self.psi = alloc((n,) + tuple(b.psi.shape), torch.float32)
 
-    def save(self, k: int, b: "_s.Batch"):
-        self.fields[k - 1].copy_(b.fields)
-        self.psi[k - 1].copy_(b.psi)
+    def save(self, step: int, b: "_s.Batch"):
+        k = self.steps.index(step)
+        self.fields[k].copy_(b.fields)
+        self.psi[k].copy_(b.psi)
 
-    def restore(self, k: int, b: "_s.Batch"):
-        if k == 0:
+    def restore(self, step: int, b: "_s.Batch"):
+        if step == 0:
             b.reset_state()
-        else:
-            b.fields.copy_(self.fields[k - 1])
-            b.psi.copy_(self.psi[k - 1])
+            return
+        k = self.steps.index(step)
+        b.fields.copy_(self.fields[k])
+        b.psi.copy_(self.psi[k])
+
+    def ez(self, step: int) -> torch.Tensor:
+        """Saved Ez (S, NI+1, pitch) of the state at ``step``: Ez after step
+        ``step`` - 1."""
+        return self.fields[self.steps.index(step)][0]
 
     def nbytes(self) -> int:
         return self.fields.numel() * 4 + self.psi.numel() * 4
+
+
+def forward_cuts(hist: History) -> list:
+    """(n0, n1, save_at_n0, store) pieces of the first forward sweep: it stops
+    where a state is saved and where the snapshots start to be kept."""
+    marks = set(hist.saves()) | {hist.first_stored(), hist.nt}
+    marks.discard(0)
+    out, n = [], 0
+    for m in sorted(marks):
+        if m <= n:
+            continue
+        out.append((n, m, n in hist.saves(), n >= hist.first_stored()))
+        n = m
+    return out
 
 
 @dataclass
 class Forward:
     batch: "_s.Batch"
     states: States
+    hist: History
+    ring: torch.Tensor
     coef: torch.Tensor
     scale: torch.Tensor
 
 
-def forward(G, er, se, mr, C: int):
-    """Forward sweep saving the state every C steps.  Returns (traces, Forward)."""
+def _ring_problem(b, coef, scale, ring, R):
+    p = b.problem(coef, srcscale=scale)
+    p.snaps, p.nsnap, p.snap_mod = ring.data_ptr(), R, R
+    return p
+
+
+def forward(G, er, se, mr, hist: History):
+    """Forward sweep keeping the last R snapshots in the ring and the states
+    the head segments restart from.  Returns (traces, Forward)."""
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
     coef = _s.cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
     b = _s.Batch(G, store=False, tb=G.time_block)
     scale = _s.emitting_sources(G, b)
-    st = States(b, C)
-    p = b.problem(coef, srcscale=scale)
-    for k in range(st.nseg):
-        if k >= 1:
-            st.save(k, b)
-        check(lib.fgpr_forward(ctypes.byref(p), k * C, min((k + 1) * C, b.nt),
+    st = States(b, hist.saves())
+    ring = torch.empty((hist.R,) + b.snap_slot_shape(), dtype=torch.float32, device=dev)
+    plain = b.problem(coef, srcscale=scale)
+    kept = _ring_problem(b, coef, scale, ring, hist.R)
+    for n0, n1, save, store in forward_cuts(hist):
+        if save:
+            st.save(n0, b)
+        check(lib.fgpr_forward(ctypes.byref(kept if store else plain), n0, n1,
                                _s._MODES[G.exec_mode], G.graph_chunk, _s._stream(dev)), "forward")
-    return b.traces, Forward(b, st, coef, scale)
+    return b.traces, Forward(b, st, hist, ring, coef, scale)
 
 
 def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True, exact=False):
-    """Adjoint sweep segment by segment with recomputed snapshots.
-
-    Same results as ``solver.adjoint`` (reference mode) or
-    ``solver.adjoint_exact`` (``exact=True``) on a full history."""
+    """Adjoint over the stored tail, then segment by segment over the head
+    with recomputed snapshots.  Same results as ``solver.adjoint``
+    (reference mode) or ``solver.adjoint_exact`` (``exact=True``) on a full
+    history."""
     dev = G.device
-    b, st = fw.batch, fw.states
-    C, nt, S, NI, NJ = st.C, b.nt, b.S, b.NI, b.NJ
+    b, st, hist, ring = fw.batch, fw.states, fw.hist, fw.ring
+    nt, S, NI, NJ, R = b.nt, b.S, b.NI, b.NJ, hist.R
     stream = _s._stream(dev)
     mode = _s._MODES[G.exec_mode]
-    ring = torch.empty((C + 1,) + b.snap_slot_shape(), dtype=torch.float32, device=dev)
     traces_scratch = torch.empty_like(b.traces)
-    pf = b.problem(fw.coef, srcscale=fw.scale)
-    pf.snaps, pf.nsnap, pf.snap_mod = ring.data_ptr(), C + 1, C + 1
+    pf = _ring_problem(b, fw.coef, fw.scale, ring, R)
     pf.traces = traces_scratch.data_ptr()
 
     ba = _s.Batch(G, store=False, tb=None if exact else G.time_block)
-    if exact:
-        coef_a = fw.coef
-    else:
-        coef_a = _s.cell_coeffs(er, se, mr, G.dt32, G.dx32, True, dev)      # Ucgeta
+    coef_a = fw.coef if exact else _s.cell_coeffs(er, se, mr, G.dt32, G.dx32, True, dev)  # Ucgeta
     resid = grad_output.detach().to(device=dev, dtype=torch.float32).contiguous()
     if resid.shape != (S, nt, b.nrx):
         raise ValueError(f"grad_output shape {tuple(resid.shape)} != {(S, nt, b.nrx)}")
@@ -142,7 +269,7 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
     if ba.tb:
         ba.gshot_buffer()
     pa = ba.problem(coef_a, resid=resid, grad_eps=ge, grad_sig=gs)
-    pa.snaps, pa.nsnap, pa.snap_mod = ring.data_ptr(), C + 1, C + 1
+    pa.snaps, pa.nsnap, pa.snap_mod = ring.data_ptr(), R, R
     scratch = None
     if exact:
         scratch = torch.zeros((7, S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
@@ -150,19 +277,21 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
         pa.tb_k = 0
     gmode = _native.GRAD_EXACT if exact else _native.GRAD_REFERENCE
 
-    for k in reversed(range(st.nseg)):
-        s0, e = k * C, min((k + 1) * C, nt)
-        st.restore(k, b)
-        if k >= 1:
-            # Ez after step s0 - 1 is the saved state's Ez
-            b.seed_slot(pf, (s0 - 1) % (C + 1), ring, st.fields[k - 1][0])
+    def adj(n_hi, n_lo):
+        if ba.tb:
+            ba.gshot.zero_()
+        check(lib.fgpr_adjoint(ctypes.byref(pa), n_hi, n_lo, gmode, mode, G.graph_chunk, stream),
+              "adjoint")
+
+    t = hist.tail()
+    if t is not None:
+        adj(*t)
+    for s0, e in reversed(hist.segments()):
+        st.restore(s0, b)
+        if s0 >= 1:
+            b.seed_slot(pf, (s0 - 1) % R, ring, st.ez(s0))      # Ez after step s0 - 1
         check(lib.fgpr_forward(ctypes.byref(pf), s0, e, mode, G.graph_chunk, stream), "recompute")
-        n_lo = s0 if exact else max(s0, 1)
-        if e - 1 >= n_lo:
-            if ba.tb:
-                ba.gshot.zero_()
-            check(lib.fgpr_adjoint(ctypes.byref(pa), e - 1, n_lo, gmode, mode, G.graph_chunk,
-                                   stream), "adjoint")
+        adj(e - 1, max(s0, hist.lowest))
     if exact:
         return _s.finish_exact(G, ge, gs, want_er, want_se)
     return _s.finish_reference(G, ge, gs, want_er, want_se)

@@ -20,13 +20,15 @@ Extra keyword arguments (all default to the reference behaviour):
                      gradient) or "exact" (dL/d er, dL/d se of the discrete
                      forward map through its exact adjoint, unnormalised).
 * ``checkpoint``  -- None (store every snapshot, as the reference), an
-                     interval C (keep the solver state every C steps and
-                     recompute snapshots segment by segment in the adjoint),
-                     or "auto" (the default: store every snapshot, as the
+                     interval C (keep the last C + 1 snapshots, save the
+                     solver state every C steps before them and recompute
+                     those snapshots segment by segment in the adjoint), or
+                     "auto" (the default: store every snapshot, as the
                      reference, unless the history would not fit in the free
                      device memory less a reserve for the rest of the
-                     gradient; checkpointed gradients are bitwise the stored
-                     ones).
+                     gradient; then keep as many of the last snapshots as
+                     fit and recompute only the earlier ones).  Checkpointed
+                     gradients are bitwise the stored ones.
 * ``shot_streams`` -- shot groups of a time-blocked sweep, each on its own
                      HIP stream so that the launches of one fill the CUs the
                      other's launch tail leaves idle (default 3; 1 = one
@@ -211,10 +213,10 @@ class GPRFWI(torch.autograd.Function):
         ctx.save_for_backward(er, se, mr)
         ctx.G = G
         need_grad = er.requires_grad or se.requires_grad
-        C = _ckpt.interval(G, G.options.get("checkpoint")) if need_grad else None
+        hist = _ckpt.history(G, G.options.get("checkpoint")) if need_grad else None
         ctx.fw = None
-        if C:
-            traces, ctx.fw = _ckpt.forward(G, er, se, mr, C)
+        if hist is not None and not hist.full:
+            traces, ctx.fw = _ckpt.forward(G, er, se, mr, hist)
             ctx.batch = None
         else:
             traces, batch = _solver.forward(G, er, se, mr, store=need_grad)

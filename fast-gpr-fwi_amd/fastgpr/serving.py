@@ -2,22 +2,37 @@
 inversions, continuously batched on one GPU.
 
 * Requests arrive on a thread-safe queue (``submit``) with their own grid,
-  shots, acquisition, current model and observed traces.
-* Admission is by device bytes: a request is admitted when its estimated
-  footprint (``batching.job_bytes``) fits in what the budget has left;
-  histories that would take more than a quarter of the budget are
-  checkpointed (interval ~ sqrt(3 nt)).
+  shots, acquisition, current model and observed traces; ``cancel`` withdraws
+  one, waiting or running.
+* Device memory is one arena the server allocates at start
+  (``batching.Arena``); every job's large buffers are carved from one block
+  of it, so admission is exact: a request is admitted when a free block
+  holds its footprint (``batching.job_bytes``), and nothing else competes
+  for that memory.
+* The footprint adapts to the memory free at admission: a gradient keeps
+  its whole snapshot history when that fits in half of the largest free
+  block (all of it when nothing else runs), otherwise the largest snapshot
+  ring that does (``checkpoint.History``: only the steps before the ring are
+  recomputed).  A request whose smallest footprint does not fit waits;
+  one that waited longer than ``max_hold_s`` holds back every younger
+  request until it is admitted, so a stream of small requests cannot starve
+  a large one.
 * Admitted requests become ``batching.Job``s; every tick one multi-problem
-  launch per (variant, direction) advances every active job by one block, so
-  requests that arrive while others run join at the next tick and their work
-  overlaps on the device.  Models, CPML state, snapshot rings and
-  checkpoints stay resident until the request completes.
+  launch per (stream, variant, direction) advances every active job by one
+  block, so requests that arrive while others run join at the next tick and
+  their work overlaps on the device.  Models, CPML state, snapshot rings and
+  saved states stay resident until the request completes.
+* A request that cannot be served -- invalid input, larger than the arena,
+  a non-finite trace or gradient -- fails alone: its result is
+  ``{"error": ...}`` and every other request goes on.
 * A request is complete when its final event has passed on the device; the
   loop records its latency (arrival to completion) and the cell updates it
   did.  Results do not depend on the mix: each job owns its buffers and the
   kernels do the same arithmetic alone or batched.
 
-No counterpart in the reference (one model per call, compute.py:33-50).
+No counterpart in the reference (one model per call, compute.py:33-50); its
+only guards are the NaN prints of compute.py:888-889 and the adjoint loop's
+stop at a NaN field (compute.py:1049-1054).
 """
 from __future__ import annotations
 
@@ -31,6 +46,7 @@ import numpy as np
 import torch
 
 from . import batching
+from .checkpoint import History
 from .solver import to_device
 from .workloads import Workload
 
@@ -46,13 +62,15 @@ class Request:
     result: dict | None = None
     cell_updates: int = 0
     tb: tuple | None = None
-    checkpoint: int | None = None
+    history: History | None = None
+    recompute_steps: int = 0
 
 
 @dataclass
 class Stats:
     completed: int = 0
     cell_updates: int = 0
+    recompute_cell_updates: int = 0
     latencies: list = field(default_factory=list)
     started: float = 0.0
     finished: float = 0.0
@@ -61,9 +79,10 @@ class Stats:
     max_active: int = 0
     ticks: int = 0
     tick_host_s: float = 0.0        # host time spent packing and launching ticks
-    cache_releases: int = 0         # admissions that fitted only after releasing torch's cache
-    oom_retries: int = 0            # admissions put back because the device ran out of memory
-    refused: int = 0                # requests refused (larger than the budget, or no memory)
+    holds: int = 0                  # admissions put off because no free block held the request
+    refused: int = 0                # requests that failed (invalid, too large, non-finite)
+    cancelled: int = 0
+    partial_histories: int = 0      # gradients admitted with a snapshot ring shorter than nt
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -72,34 +91,45 @@ class Stats:
                 "p50_latency_s": float(np.percentile(lat, 50)),
                 "p99_latency_s": float(np.percentile(lat, 99)),
                 "sustained_gcell_updates_per_s": self.cell_updates / wall / 1e9,
+                "recompute_share": self.recompute_cell_updates / max(
+                    self.cell_updates + self.recompute_cell_updates, 1),
                 "wall_s": wall, "launches": self.launches, "ticks": self.ticks,
                 "host_ms_per_tick": 1e3 * self.tick_host_s / max(self.ticks, 1),
-                "cache_releases": self.cache_releases, "oom_retries": self.oom_retries,
-                "refused": self.refused,
+                "holds": self.holds, "refused": self.refused, "cancelled": self.cancelled,
+                "partial_histories": self.partial_histories,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
 
 
 class Server:
-    """Continuous-batching gradient server for one device."""
+    """Continuous-batching gradient server for one device.
+
+    ``hbm_budget``: bytes of the arena (default 85 % of the free memory);
+    ``variant_for(wl)``: the time-blocking variant of a request without one.
+    ``scheduler`` and ``arena`` may be given (tests)."""
 
     def __init__(self, device="cuda:0", hbm_budget: int | None = None, variant_for=None,
-                 max_inflight_ticks: int = 64):
+                 max_inflight_ticks: int = 64, max_hold_s: float = 2.0, scheduler=None,
+                 arena=None):
         self.device = torch.device(device)
-        if hbm_budget is None:
-            free, _ = torch.cuda.mem_get_info(self.device)
-            hbm_budget = int(0.7 * free)
-        self.budget = hbm_budget
+        if arena is None:
+            if hbm_budget is None:
+                free, _ = torch.cuda.mem_get_info(self.device)
+                hbm_budget = int(0.85 * free)
+            arena = batching.Arena(self.device, hbm_budget)
+        self.arena = arena
+        self.budget = arena.size
         self.variant_for = variant_for or (lambda wl: (64, 1024, 6))
         self.pending: "queue.Queue[Request]" = queue.Queue()
         self.waiting: list = []
-        self.active: list = []          # (request, job, bytes)
+        self.active: list = []          # (request, job, arena offset, bytes)
         self.in_use = 0
-        self.sched = batching.Scheduler(self.device)
+        self.sched = scheduler if scheduler is not None else batching.Scheduler(self.device)
         self.stats = Stats()
         self.max_inflight = max_inflight_ticks
+        self.max_hold_s = max_hold_s
         self._tick_events: list = []
+        self._cancel: set = set()
         self._lock = threading.Lock()
-        self._hold = None               # (completed count, bytes) of the last request that did not fit
 
     # --- client side -----------------------------------------------------------
     def submit(self, req: Request) -> Request:
@@ -107,20 +137,45 @@ class Server:
         self.pending.put(req)
         return req
 
+    def cancel(self, rid: int):
+        """Withdraw request ``rid``: a waiting one is dropped, a running one
+        stops at its next block (its memory is released once its queued work
+        has finished).  Its result is {"error": "cancelled"}."""
+        with self._lock:
+            self._cancel.add(rid)
+
     # --- scheduler side ----------------------------------------------------------
+    def _fail(self, req: Request, msg: str, cancelled=False):
+        req.result = {"error": msg}
+        req.done_at = time.perf_counter()
+        if cancelled:
+            self.stats.cancelled += 1
+        else:
+            self.stats.refused += 1
+
     def _plan(self, req: Request):
+        """(variant, history or None, bytes, smallest bytes) of a request
+        given the memory free now."""
         wl = req.wl
-        tb = req.tb or self.variant_for(wl)
-        K = tb[2]
-        full = batching.job_bytes(wl.NI, wl.NJ, wl.S, wl.nt, wl.rx.shape[1], wl.P, wl.nt,
-                                  req.obs is not None, tb)
-        C = wl.nt
-        if req.obs is not None and full > self.budget // 4:
-            c = int(math.ceil(math.sqrt(3.0 * wl.nt)))
-            C = max(2 * K, ((c + 2 * K - 1) // (2 * K)) * 2 * K)
-        nbytes = batching.job_bytes(wl.NI, wl.NJ, wl.S, wl.nt, wl.rx.shape[1], wl.P, C,
-                                    req.obs is not None, tb)
-        return tb, (None if C >= wl.nt else C), nbytes
+        tb = tuple(req.tb or self.variant_for(wl))
+        if tb not in batching.MULTI_VARIANTS:
+            raise ValueError(f"time-blocking variant {tb} has no multi-problem kernel")
+        spec = (wl.NI, wl.NJ, wl.S, wl.nt, int(wl.rx.shape[1]), wl.P)
+        if req.obs is None:
+            nbytes = batching.job_bytes(*spec, wl.nt, False, tb)
+            return tb, None, nbytes, nbytes
+        low = _smallest_history(spec, tb)
+        smallest = batching.job_bytes(*spec, low, True, tb)
+        full = batching.job_bytes(*spec, wl.nt, True, tb)
+        largest = self.arena.largest_free()
+        share = largest if not self.active else largest // 2
+        if full <= share:
+            return tb, History(wl.nt, wl.nt), full, smallest
+        try:
+            hist = batching.job_history(*spec, max(share, smallest), tb)
+        except MemoryError:
+            hist = low
+        return tb, hist, batching.job_bytes(*spec, hist, True, tb), smallest
 
     def _admit(self):
         while True:
@@ -128,56 +183,56 @@ class Server:
                 self.waiting.append(self.pending.get_nowait())
             except queue.Empty:
                 break
-        if self._hold is not None and self.stats.completed != self._hold[0]:
-            self._hold = None              # a request has finished: memory was freed
+        with self._lock:
+            cancel, self._cancel = self._cancel, set()
         still = []
-        for k, req in enumerate(self.waiting):
-            tb, C, nbytes = self._plan(req)
-            if self._hold is not None and nbytes >= self._hold[1]:
-                still.append(req)          # no larger than what just failed to fit may pass
+        now = time.perf_counter()
+        blocked = False                 # a long-held request blocks the younger ones
+        for req in self.waiting:
+            if req.rid in cancel:
+                self._fail(req, "cancelled", cancelled=True)
                 continue
-            if nbytes > self.budget:
-                req.result = {"error": f"request needs {nbytes} bytes, budget {self.budget}"}
-                req.done_at = time.perf_counter()
-                self.stats.refused += 1
-                continue
-            if self.in_use + nbytes > self.budget:
-                still.append(req)          # FIFO order kept; later small ones may pass
-                continue
-            job = None
-            for attempt in range(2):
-                try:
-                    job = self._make_job(req, tb, C)
-                    break
-                except torch.OutOfMemoryError:
-                    # The device holds less than the byte accounting says:
-                    # memory cached by torch in pieces too small for this
-                    # request.  Release the unused cache and try once more.
-                    torch.cuda.empty_cache()
-            if job is not None and attempt == 1:
-                self.stats.cache_releases += 1
-            if job is None:
-                # Still no room: this request, and any not smaller, wait until
-                # a running request has finished; smaller ones may still be
-                # admitted.  With nothing running, nothing will free memory:
-                # refuse the request.
-                if not self.active:
-                    req.result = {"error": "device out of memory with no request running"}
-                    req.done_at = time.perf_counter()
-                    self.stats.refused += 1
-                    continue
-                self.stats.oom_retries += 1
-                self._hold = (self.stats.completed, nbytes)
+            if blocked:
                 still.append(req)
                 continue
+            try:
+                tb, hist, nbytes, smallest = self._plan(req)
+            except Exception as e:                       # invalid request: it fails alone
+                self._fail(req, f"invalid request: {type(e).__name__}: {e}")
+                continue
+            if smallest > self.arena.size:
+                self._fail(req, f"request needs at least {smallest} bytes, arena {self.arena.size}")
+                continue
+            off = self.arena.alloc(nbytes)
+            if off is None:
+                self.stats.holds += 1
+                still.append(req)
+                if now - req.arrival > self.max_hold_s:
+                    blocked = True
+                continue
+            try:
+                job = self._make_job(req, tb, hist, self.arena.region(off))
+            except Exception as e:
+                self.arena.release(off)
+                self._fail(req, f"could not build the job: {type(e).__name__}: {e}")
+                continue
+            req.tb, req.history = tb, hist
+            if hist is not None and not hist.full:
+                self.stats.partial_histories += 1
+                req.recompute_steps = hist.recompute_steps()
             self.sched.add(job)
-            self.active.append((req, job, nbytes))
+            self.active.append((req, job, off, nbytes))
             self.in_use += nbytes
             self.stats.peak_bytes = max(self.stats.peak_bytes, self.in_use)
         self.waiting = still
+        for req, job, _, _ in self.active:
+            if req.rid in cancel and not job.done:
+                job.cancel()
+                req.result = {"error": "cancelled"}
+                self.sched.mark_done(job)            # its event follows the job's queued work
         self.stats.max_active = max(self.stats.max_active, len(self.active))
 
-    def _make_job(self, req: Request, tb, C):
+    def _make_job(self, req: Request, tb, hist, region):
         import fastgpr
         wl = req.wl
         dev = self.device
@@ -192,23 +247,36 @@ class Server:
                                    er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
                                    False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
         obs = None if req.obs is None else to_device(torch.as_tensor(req.obs), dev)
-        req.tb, req.checkpoint = tb, C
-        return batching.Job(G, er, se, mr, obs=obs, tb=tb, checkpoint=C, tag=req.rid)
+        job = batching.Job(G, er, se, mr, obs=obs, tb=tb, history=hist, tag=req.rid, alloc=region,
+                           keep=lambda shape, dt: torch.empty(shape, dtype=dt, device=dev))
+        job.finite_check = True
+        return job
 
     def _reap(self):
         keep = []
-        for req, job, nbytes in self.active:
+        for req, job, off, nbytes in self.active:
             if job.done and job.done_event is not None and job.done_event.query():
+                self.arena.release(off)
+                self.in_use -= nbytes
+                self.sched.jobs.remove(job)
                 req.done_at = time.perf_counter()
-                req.result = job.result
+                if job.cancelled:
+                    self.stats.cancelled += 1
+                    continue
+                res = job.results()
+                if not job.finite():
+                    self._fail(req, "non-finite traces or gradient (the reference prints "
+                                    "'contains NaN' here)")
+                    continue
+                req.result = res
                 req.cell_updates = job.cell_updates()
                 self.stats.completed += 1
                 self.stats.cell_updates += req.cell_updates
+                self.stats.recompute_cell_updates += (req.recompute_steps * req.wl.S * req.wl.NI
+                                                      * req.wl.NJ)
                 self.stats.latencies.append(req.done_at - req.arrival)
-                self.in_use -= nbytes
-                self.sched.jobs.remove(job)
             else:
-                keep.append((req, job, nbytes))
+                keep.append((req, job, off, nbytes))
         self.active = keep
 
     def step(self) -> bool:
@@ -224,12 +292,7 @@ class Server:
             self.stats.tick_host_s += time.perf_counter() - t
             self.stats.ticks += 1
             self.stats.launches += self.sched.launches - before
-            evs = []
-            for st in self.sched.streams:
-                ev = torch.cuda.Event()
-                ev.record(st)
-                evs.append(ev)
-            self._tick_events.append(evs)
+            self._tick_events.append(self.sched.tick_events())
             if len(self._tick_events) > self.max_inflight:
                 for ev in self._tick_events.pop(0):        # bound how far the host runs ahead
                     ev.synchronize()
@@ -249,6 +312,17 @@ class Server:
         self.sched.join()
         self.stats.finished = time.perf_counter()
         return self.stats.summary()
+
+
+def _smallest_history(spec, tb) -> History:
+    """The ring that minimises a gradient's snapshot footprint: about
+    sqrt(nt * state / slot) slots."""
+    NI, NJ, S, nt, nrx, P = spec
+    slot, state = batching._slot_state(NI, NJ, S, P, tb)
+    R = max(2, min(nt, int(round(math.sqrt(nt * state / max(slot, 1))))))
+    best = min((History(nt, r) for r in range(max(2, R // 2), min(nt, 2 * R) + 1)),
+               key=lambda h: h.nbytes(slot, state))
+    return best
 
 
 def poisson_trace(rate_hz: float, duration_s: float, seed: int = 4) -> np.ndarray:

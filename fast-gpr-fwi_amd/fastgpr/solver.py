@@ -46,13 +46,17 @@ def require_hip(device) -> torch.device:
     return device
 
 
-def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device) -> torch.Tensor:
-    """5 x (NI+1) x pitch coefficient planes (Ucget / Ucgeta, uc.cu:48, 94)."""
+def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device,
+                out=None) -> torch.Tensor:
+    """5 x (NI+1) x pitch coefficient planes (Ucget / Ucgeta, uc.cu:48, 94),
+    into ``out`` when given.  Contiguous float32 device inputs are read in
+    place (nothing is allocated)."""
     NI, NJ = er.shape[0], er.shape[1]
     pitch = _native.pitch(NJ)
     e, s, m = (to_device(x.detach().reshape(NI, NJ).to(torch.float32), device).contiguous()
                for x in (er, se, mr))
-    coef = torch.empty((5, NI + 1, pitch), dtype=torch.float32, device=device)
+    coef = (torch.empty((5, NI + 1, pitch), dtype=torch.float32, device=device)
+            if out is None else out)
     check(lib.fgpr_cell_coeffs(_ptr(e), _ptr(s), _ptr(m), NI, NJ, dt32, dx32,
                                1 if adjoint else 0, _ptr(coef), _stream(device)), "cell_coeffs")
     return coef
@@ -112,18 +116,27 @@ class Batch:
     ``tb`` = (region rows, threads, steps per launch) selects the
     time-blocked kernels; None selects the per-step kernels.  ``shots`` =
     (lo, hi) restricts the batch to shots lo..hi-1 of ``G`` (a shot group);
-    ``traces`` may then be the group's slice of a whole-batch tensor."""
+    ``traces`` may then be the group's slice of a whole-batch tensor.
 
-    def __init__(self, G, store: bool = True, tb=None, tiled=None, shots=None, traces=None):
+    ``alloc(shape, dtype)`` supplies the large buffers (fields, CPML memory,
+    traces, snapshots) uninitialised -- a caller carving them from its own
+    device arena then clears them with ``reset_state`` and ``traces.zero_()``
+    in stream order; by default they are zeroed torch tensors."""
+
+    def __init__(self, G, store: bool = True, tb=None, tiled=None, shots=None, traces=None,
+                 alloc=None):
         dev = G.device
         self.G = G
+        self.alloc = alloc
+        zeros = ((lambda shape, dt: torch.zeros(shape, dtype=dt, device=dev)) if alloc is None
+                 else alloc)
         self.lo, self.hi = shots if shots is not None else (0, G.S)
         self.NI, self.NJ, self.S, self.nt = G.NI, G.NJ, self.hi - self.lo, G.iterations
         self.P = G.pmlthick
         self.pitch = _native.pitch(self.NJ)
-        self.fields = torch.zeros((3, self.S, self.NI + 1, self.pitch), dtype=torch.float32, device=dev)
+        self.fields = zeros((3, self.S, self.NI + 1, self.pitch), torch.float32)
         npsi = max(_native.psi_floats(self.NI, self.NJ, self.P, self.S), 1)
-        self.psi = torch.zeros(npsi, dtype=torch.float32, device=dev)
+        self.psi = zeros((npsi,), torch.float32)
         self.pml = (to_device(G.pml, dev).contiguous() if self.P > 0
                     else torch.zeros(8, dtype=torch.float32, device=dev))
         from .grid import device_cells
@@ -136,7 +149,7 @@ class Batch:
         self.nrx = self.rx.shape[1]
         self.ctr = torch.zeros(1, dtype=torch.int32, device=dev)
         self.traces = (traces if traces is not None else
-                       torch.zeros((self.S, self.nt, self.nrx), dtype=torch.float32, device=dev))
+                       zeros((self.S, self.nt, self.nrx), torch.float32))
         self.tb = tuple(tb) if tb else None
         # Snapshots in the time-blocked layout when both sweeps run the same
         # time-blocked variant (the exact adjoint reads the row-major one).
@@ -145,21 +158,22 @@ class Batch:
         self.tiled = bool(tiled) and bool(self.tb)
         self.gshot = None
         if self.tb:
-            self.fields2 = torch.zeros_like(self.fields)
-            self.psi2 = torch.zeros_like(self.psi)
+            self.fields2 = zeros(tuple(self.fields.shape), torch.float32)
+            self.psi2 = zeros(tuple(self.psi.shape), torch.float32)
             t = tile_tables(rx_host.numpy(), self.NI, self.NJ, self.tb)
             self.tiles = (t["ti"], t["tj"])
             as_dev = lambda a: to_device(torch.from_numpy(np.ascontiguousarray(a)), dev)
             self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
                               as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
-        self.snaps = (torch.empty((self.nt,) + self.snap_slot_shape(), dtype=torch.float32, device=dev)
-                      if store else None)
+        empty = ((lambda shape, dt: torch.empty(shape, dtype=dt, device=dev)) if alloc is None
+                 else alloc)
+        self.snaps = empty((self.nt,) + self.snap_slot_shape(), torch.float32) if store else None
 
     def snap_slot_shape(self) -> tuple:
         """Shape of one snapshot slot: (S, NI, pitch) row-major, or
-        (S, tiles * rows * 64) in the time-blocked layout."""
+        (S, floats per shot) in the time-blocked layout."""
         if self.tiled:
-            return (self.S, self.tiles[0] * self.tiles[1] * self.tb[0] * 64)
+            return (self.S, _native.tb_snap_floats(self.NI, self.NJ, self.tb))
         return (self.S, self.NI, self.pitch)
 
     def seed_slot(self, p, slot: int, ring: torch.Tensor, ez: torch.Tensor):
@@ -177,13 +191,14 @@ class Batch:
             self.fields2.zero_()
             self.psi2.zero_()
 
-    def gshot_buffer(self):
+    def gshot_buffer(self, zero: bool = True):
         """Per-shot FP64 partial gradients of the time-blocked adjoint,
-        (S, 2, NI, NJ), zeroed."""
+        (S, 2, NI, NJ), zeroed unless ``zero`` is false."""
         if self.gshot is None:
-            self.gshot = torch.zeros((self.S, 2, self.NI, self.NJ), dtype=torch.float64,
-                                     device=self.fields.device)
-        else:
+            shape = (self.S, 2, self.NI, self.NJ)
+            self.gshot = (torch.empty(shape, dtype=torch.float64, device=self.fields.device)
+                          if self.alloc is None else self.alloc(shape, torch.float64))
+        if zero:
             self.gshot.zero_()
         return self.gshot
 

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 9
+#define FGPR_ABI_VERSION 10
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -125,8 +125,9 @@ typedef struct fgpr_problem {
     int32_t snap_mod;           /* > 0: step n lives in slot n % snap_mod (a ring of segment
                                    slots; the same captured graph then serves every segment) */
     int32_t snap_tiled;         /* 1: snapshots in the time-blocked layout of this problem's
-                                   variant -- per (slot, shot, tile) a block of tb_ri x 64 floats
-                                   ordered [wave][lane][row] -- written by the time-blocked
+                                   variant -- per (slot, shot, tile) a block of the waves owning
+                                   tile-centre rows x the tile-centre lanes x the rows of a
+                                   thread, ordered [wave][lane][row] -- written by the time-blocked
                                    forward and read by the time-blocked adjoint of the same
                                    variant; 0: the row-major layout above */
 } fgpr_problem;
@@ -264,8 +265,30 @@ int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_ite
 int fgpr_row_sums(const float* a, const float* b, int n, int stride, float* out, void* stream);
 
 /* Floats per snapshot slot in the problem's layout (row-major S*NI*pitch, or
- * S*tiles*tb_ri*64 when snap_tiled). */
+ * S * fgpr_tb_snap_floats(...) when snap_tiled). */
 int64_t fgpr_snap_slot_floats(const fgpr_problem* p);
+
+/* Floats per shot of one snapshot slot in the time-blocked layout of variant
+ * (tb_ri, tb_nt, tb_k) on an NI x NJ grid: per tile, the waves that own
+ * tile-centre rows times the tile-centre lanes times the rows per thread
+ * (about 1.1x NI*NJ for the 64-row variants); -1 for a variant that is not
+ * compiled in.  No counterpart in the reference. */
+int64_t fgpr_tb_snap_floats(int NI, int NJ, int tb_ri, int tb_nt, int tb_k);
+
+/* The work-item mapping of one time-blocked launch of p (adj = 0 forward,
+ * 1 adjoint), evaluated on the host by the code the kernels run: for every
+ * workgroup wg < S * tiles, its shot, its tile and whether it is one of the
+ * interior items the launch orders first.  Returns the workgroup count or a
+ * negative code.  No GPU work; no counterpart in the reference (inspection
+ * and tests). */
+int64_t fgpr_tb_map(const fgpr_problem* p, int adj, int32_t* shot, int32_t* tile, int8_t* interior,
+                    int64_t n);
+
+/* Count of dividends n in [n0, n1) (0 <= n0 <= n1 <= 2^31) for which the
+ * kernels' multiply-and-shift division by d (FastDiv, evaluated on the host)
+ * differs from n / d; -1 for bad arguments.  No counterpart in the
+ * reference (tests). */
+int64_t fgpr_fastdiv_mismatches(int d, int64_t n0, int64_t n1);
 
 /* Writes slot `slot` of a tiled snapshot buffer from a row-major Ez
  * (S*(NI+1)*pitch floats, the layout of p->Ez): seeds the ring slot before a

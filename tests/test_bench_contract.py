@@ -32,6 +32,21 @@ def test_defaults():
     import bench
     a = bench.parse([])
     assert (a.gpus, a.steps, a.warmup, a.shot_streams, a.time_block) == (1, 5, 1, 3, "auto")
+    assert a.workload == "c3" and not a.quick
+
+
+def test_measured_traffic_is_keyed_by_workload_and_kernel(tmp_path):
+    """Counter bytes are reported only for the exact workload and kernel the
+    counter runs measured."""
+    import json
+    import bench
+    f = tmp_path / "traffic.json"
+    f.write_text(json.dumps({"measured": {"c2_crosshole_400x200_13shots|k_tb<64,1024,6,true>":
+                                          {"bytes_per_launch": 123}}}))
+    assert bench.measured_traffic(f, "c2_crosshole_400x200_13shots|k_tb<64,1024,6,true>") == 123
+    assert bench.measured_traffic(f, "c2_crosshole_400x200_13shots|k_tb<64,1024,8,true>") is None
+    assert bench.measured_traffic(f, "other|k_tb<64,1024,6,true>") is None
+    assert bench.measured_traffic(tmp_path / "missing.json", "x|y") is None
 
 
 def test_kernel_busy_takes_the_union_inside_the_window(tmp_path):

@@ -77,3 +77,23 @@ def test_c_fp32_mode_is_bitwise_numpy_fp32(P):
 
 def test_c_oracle_reports_threads():
     assert CO.threads() >= 1
+
+
+def test_sanitized_oracle_stays_in_bounds():
+    """AddressSanitizer + UndefinedBehaviorSanitizer build of the C
+    restatement (both precisions) over small grids with sources and
+    receivers on corners and the padding row / column: no report.  A planted
+    receiver past the fields is caught (the check has teeth)."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+    if shutil.which("gcc") is None or shutil.which("make") is None:
+        pytest.skip("gcc / make not available")
+    here = Path(__file__).resolve().parents[1] / "oracle"
+    subprocess.run(["make", "-s", "-C", str(here), "sanitize"], check=True, timeout=300)
+    for exe in ("sanitize_check", "sanitize_check_f32"):
+        r = subprocess.run([str(here / "_build" / exe)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "Sanitizer" not in r.stderr, r.stderr[-2000:]
+    r = subprocess.run([str(here / "_build" / "sanitize_check"), "oob"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode != 0 and "AddressSanitizer: heap-buffer-overflow" in r.stderr

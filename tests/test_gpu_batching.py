@@ -47,7 +47,7 @@ def test_mixed_jobs_equal_alone_and_compute(hip_device):
         jobs = []
         for (wl, tb, ck), o, init in zip(cases, obs, inits):
             G, er, se, mr = _job_inputs(wl, dev, init)
-            jobs.append(Job(G, er, se, mr, obs=o, tb=tb, checkpoint=ck))
+            jobs.append(Job(G, er, se, mr, obs=o, tb=tb, history=ck))
         return jobs
 
     mixed = make_jobs()
@@ -63,14 +63,14 @@ def test_mixed_jobs_equal_alone_and_compute(hip_device):
         s1.add(alone)
         s1.run()
         torch.cuda.synchronize()
-        resid = (mixed[idx].result["traces"] - o).cpu().numpy()
+        resid = (mixed[idx].results()["traces"] - o).cpu().numpy()
         _, g = run_product(wl, dev, resid=resid, model=init, time_block=tb)
         for k in ("er", "se"):
-            a = mixed[idx].result[k].cpu().numpy()[..., 0]
-            b = alone.result[k].cpu().numpy()[..., 0]
+            a = mixed[idx].results()[k].cpu().numpy()[..., 0]
+            b = alone.results()[k].cpu().numpy()[..., 0]
             assert np.array_equal(a, b), (idx, k, "mix vs alone")
             assert np.array_equal(a, g[k]), (idx, k, "mix vs compute()")
-        assert torch.equal(mixed[idx].result["traces"], alone.result["traces"])
+        assert torch.equal(mixed[idx].results()["traces"], alone.results()["traces"])
 
 
 def test_forward_only_jobs_in_the_mix(hip_device):
@@ -87,4 +87,77 @@ def test_forward_only_jobs_in_the_mix(hip_device):
     torch.cuda.synchronize()
     for (wl, tb, ck), j in zip(cases, jobs):
         tr, _ = run_product(wl, dev, time_block=tb)
-        assert np.array_equal(j.result["traces"].cpu().numpy(), tr)
+        assert np.array_equal(j.results()["traces"].cpu().numpy(), tr)
+
+
+def test_captured_schedule_equals_eager_ticks(hip_device):
+    """The same job set run tick by tick from the host and replayed from one
+    captured graph (twice, with eager work between the replays) gives
+    bitwise the same traces and gradients."""
+    from fastgpr.batching import Job, Scheduler
+    dev = hip_device
+    cases = _cases()
+    obs = [torch.tensor(run_product(wl, dev, time_block=tb)[0]).to(dev) for wl, tb, _ in cases]
+    inits = [(wl.er * 0.97, wl.se * 1.1) for wl, _, _ in cases]
+
+    def make_jobs():
+        out = []
+        for (wl, tb, ck), o, init in zip(cases, obs, inits):
+            G, er, se, mr = _job_inputs(wl, dev, init)
+            out.append(Job(G, er, se, mr, obs=o, tb=tb, history=ck))
+        return out
+
+    eager = make_jobs()
+    sch = Scheduler(dev)
+    for j in eager:
+        sch.add(j)
+    sch.run()
+    torch.cuda.synchronize()
+    ref = [{k: v.cpu().clone() for k, v in j.results().items()} for j in eager]
+
+    cap_jobs = make_jobs()
+    s2 = Scheduler(dev)
+    for j in cap_jobs:
+        s2.add(j, start=False)
+    cb = s2.capture()
+    print("captured ticks", cb.ticks, "launches", cb.launches, flush=True)
+    for rep in range(2):
+        cb.replay()
+        torch.cuda.synchronize()
+        torch.zeros(1 << 20, device=dev).sum().item()        # eager work between replays
+        for j, r in zip(cap_jobs, ref):
+            for k, v in j.results().items():
+                assert torch.equal(v.cpu(), r[k]), (rep, k)
+
+
+def test_uneven_batch_waves_reuse_memory_and_match_compute(hip_device):
+    """A budget that holds one job's whole history but not all three: the
+    jobs run in waves carved from one buffer (a wave reuses the memory of the
+    one before) and a job too large alone keeps a shorter snapshot ring; every
+    result is bitwise compute()'s for that model alone, on every replay."""
+    from fastgpr.batching import job_bytes
+    from fastgpr.uneven import Problem, UnevenBatch, plan_waves
+    dev = hip_device
+    cases = _cases()
+    obs = [torch.tensor(run_product(wl, dev, time_block=tb)[0]).to(dev) for wl, tb, _ in cases]
+    inits = [(wl.er * 0.97, wl.se * 1.1) for wl, _, _ in cases]
+    specs = [(wl.NI, wl.NJ, wl.S, wl.nt, wl.rx.shape[1], wl.P, tb) for wl, tb, _ in cases]
+    full = [job_bytes(*sp[:6], sp[3], True, sp[6]) for sp in specs]
+    budget = int(0.8 * max(full))              # the largest job cannot keep its whole history
+    waves, hists = plan_waves(specs, budget)
+    assert len(waves) >= 2 and any(not h.full for h in hists), (waves, hists)
+    probs = []
+    for (wl, tb, _), o, init in zip(cases, obs, inits):
+        G, er, se, mr = _job_inputs(wl, dev, init)
+        probs.append(Problem(G, er, se, mr, o, tb))
+    ub = UnevenBatch(dev, probs, budget=budget)
+    print(ub.summary(), flush=True)
+    for rep in range(2):
+        res = ub.run()
+        torch.cuda.synchronize()
+        for idx, ((wl, tb, _), o, init) in enumerate(zip(cases, obs, inits)):
+            resid = (res[idx]["traces"] - o).cpu().numpy()
+            tr, g = run_product(wl, dev, resid=resid, model=init, time_block=tb)
+            assert np.array_equal(res[idx]["traces"].cpu().numpy(), tr), (rep, idx)
+            for k in ("er", "se"):
+                assert np.array_equal(res[idx][k].cpu().numpy()[..., 0], g[k]), (rep, idx, k)

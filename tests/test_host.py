@@ -125,7 +125,8 @@ def test_shot_groups_are_held_to_spare_hardware_queues(monkeypatch):
 def test_auto_checkpoint_counts_cached_memory_and_keeps_a_reserve(monkeypatch):
     """checkpoint="auto" stores the history when it fits in the free memory
     plus what torch's allocator caches unused, less the reserve; otherwise it
-    checkpoints (device queries stubbed: no GPU here)."""
+    keeps the largest snapshot ring that fits (device queries stubbed: no GPU
+    here)."""
     from types import SimpleNamespace
     import torch
     from fastgpr import _native, checkpoint as ck
@@ -135,14 +136,91 @@ def test_auto_checkpoint_counts_cached_memory_and_keeps_a_reserve(monkeypatch):
     monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (100 * GiB, 288 * GiB))
     monkeypatch.setattr(torch.cuda, "memory_allocated", lambda d: 10 * GiB)
     G = SimpleNamespace(iterations=1500, S=13, NI=1600, NJ=800, time_block=None, options={},
-                        graph_chunk=32, device="cuda:0")
-    full = 1500 * 13 * 1600 * _native.pitch(800) * 4
+                        graph_chunk=32, device="cuda:0", pmlthick=20)
+    slot = 13 * 1600 * _native.pitch(800) * 4
+    full = 1500 * slot
     assert 72 * GiB < full < 100 * GiB
     monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 150 * GiB)
-    assert ck.interval(G, "auto") is None                 # 100 + 140 cached - 14.4 reserve
+    assert ck.history(G, "auto") is None                  # 100 + 140 cached - 14.4 reserve
     monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 10 * GiB)
-    C = ck.interval(G, "auto")                             # 100 - 14.4 < full
-    assert C is not None and C % 32 == 0 and C >= int(np.ceil(np.sqrt(3 * 1500)))
+    h = ck.history(G, "auto")                              # 100 - 14.4 < full
+    budget = int(100 * GiB - 0.05 * 288 * GiB)
+    assert h is not None and not h.full
+    assert h.nbytes(slot, ck.state_bytes(G)) <= budget < (h.R + 1) * slot
+    assert h.recompute_steps() == h.h < 1500 // 2           # most of the history is kept
     G.NI = 400                                             # a quarter of the device: no query
     monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (_ for _ in ()).throw(AssertionError))
-    assert ck.interval(G, "auto") is None
+    assert ck.history(G, "auto") is None
+
+
+def _simulate_history(hist):
+    """Replay a History's schedule on the host: ring contents per slot, the
+    saved states, and every adjoint step with the snapshots it reads."""
+    from fastgpr.checkpoint import forward_cuts
+    nt, R = hist.nt, hist.R
+    ring = [None] * R
+    saved = {0}
+    covered = []
+    n = 0
+    for n0, n1, save, store in forward_cuts(hist):
+        assert n0 == n
+        if save:
+            saved.add(n0)
+        for k in range(n0, n1):
+            if store:
+                ring[k % R] = k
+        n = n1
+    assert n == nt
+
+    def adjoint(hi, lo):
+        for k in range(hi, lo - 1, -1):
+            assert ring[k % R] == k, (k, ring[k % R])
+            if k >= 1:
+                assert ring[(k - 1) % R] == k - 1
+            covered.append(k)
+    t = hist.tail()
+    if t is not None:
+        adjoint(*t)
+    for s0, e in reversed(hist.segments()):
+        assert s0 in saved and e - s0 <= hist.C
+        if s0 >= 1:
+            ring[(s0 - 1) % R] = s0 - 1                    # seeded from the saved Ez
+        for k in range(s0, e):
+            ring[k % R] = k
+        adjoint(e - 1, max(s0, hist.lowest))
+    assert covered == list(range(nt - 1, hist.lowest - 1, -1))
+    assert sorted(saved - {0}) == hist.saves()
+
+
+@pytest.mark.parametrize("nt,R,lowest", [(157, 38, 1), (157, 158, 1), (157, 157, 1), (100, 2, 1),
+                                         (100, 3, 0), (1200, 65, 0), (8000, 97, 0), (6000, 4321, 1),
+                                         (50, 50, 0), (50, 51, 0), (2, 2, 1), (1, 2, 0)])
+def test_history_schedule_reads_the_right_snapshots(nt, R, lowest):
+    """Every adjoint step runs once, in descending order, with its two
+    snapshots in the ring; recompute segments restart from saved states."""
+    from fastgpr.checkpoint import History
+    hist = History(nt, R, lowest)
+    _simulate_history(hist)
+    assert hist.recompute_steps() <= max(nt - R + 1, 0)
+
+
+def test_history_plan_takes_the_largest_ring_that_fits():
+    from fastgpr.checkpoint import plan
+    slot, state = 1000, 3300
+    for nt in (10, 157, 3000):
+        assert plan(nt, slot, state, nt * slot).full
+        for budget in (nt * slot - 1, nt * slot // 2, nt * slot // 7, 40_000):
+            try:
+                h = plan(nt, slot, state, budget)
+            except MemoryError:
+                from fastgpr.checkpoint import History
+                assert all(History(nt, R).nbytes(slot, state) > budget for R in range(2, nt + 1))
+                continue
+            assert h.nbytes(slot, state) <= budget
+            if h.R < nt:
+                from fastgpr.checkpoint import History
+                bigger = History(nt, h.R + 1)
+                assert bigger.nbytes(slot, state) > budget
+            _simulate_history(h)
+    with pytest.raises(MemoryError):
+        plan(3000, slot, state, 3 * slot)
