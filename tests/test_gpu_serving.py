@@ -19,7 +19,8 @@ def _requests(n, seed=11, gradient=True):
 
 def _alone(req, dev):
     from fastgpr.serving import Request, Server
-    srv = Server(dev, hbm_budget=1 << 34)
+    torch.cuda.empty_cache()
+    srv = Server(dev)
     r = srv.submit(Request(req.rid, req.wl, req.model, req.obs, tb=req.tb))
     srv.run()
     torch.cuda.synchronize()
