@@ -1817,6 +1817,31 @@ int64_t tv_bytes(int M, int N) {
     return 7 * pl + 256 + tv_blocks(M, N) * 8;
 }
 
+// Workspace of fgpr_tv_static: a file-scope device array (each device the
+// library runs on holds its own copy until the process ends).  Host code
+// never passes the array's name to a kernel: its device address, per
+// device, comes from hipGetSymbolAddress on the first call on that device.
+static __device__ __align__(256) unsigned char scratch[FGPR_TV_STATIC_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+
+int scratch_address(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(FGPR_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return FGPR_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2195,6 +2220,26 @@ int64_t fgpr_fastdiv_mismatches(int d, int64_t n0, int64_t n1) {
     for (int64_t n = n0; n < n1; ++n)
         bad += fdiv((int)n, f) != (int)(n / d);
     return bad;
+}
+
+int fgpr_tv_static(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+                   float* out, void* stream) {
+    if (M < 1 || N < 1) return fail(FGPR_EINVAL, "bad TV arguments");
+    const int64_t need = tv_bytes(M, N);
+    if (need > (int64_t)FGPR_TV_STATIC_BYTES)
+        return fail(FGPR_EINVAL, "fgpr_tv_static: a " + std::to_string(M) + " x " + std::to_string(N) +
+                                     " model needs " + std::to_string(need) +
+                                     " bytes of workspace, more than the static " +
+                                     std::to_string((int64_t)FGPR_TV_STATIC_BYTES) +
+                                     "; use fgpr_tv with a caller workspace");
+    void* ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    // every word of the workspace the kernels read is written first in the
+    // same call (k_tv_max: the maximum and the flags; k_tv_init: the seven
+    // planes; each X sweep: its block sums), so nothing is cleared here
+    return fgpr_tv(data, M, N, lamda, rho, num_iter, tol, ws, (int64_t)FGPR_TV_STATIC_BYTES, out,
+                   stream);
 }
 
 int64_t fgpr_snap_slot_floats(const fgpr_problem* p) {

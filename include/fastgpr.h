@@ -258,6 +258,20 @@ int64_t fgpr_tv_workspace_bytes(int M, int N);
 int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
             void* workspace, int64_t bytes, float* out, void* stream);
 
+/* fgpr_tv without a workspace argument: the workspace is a static device
+ * array of FGPR_TV_STATIC_BYTES inside the library (one per device, held
+ * until the process ends), so the call allocates nothing -- for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Models whose fgpr_tv_workspace_bytes(M, N) exceeds the static size (about
+ * 2.39 million padded cells, e.g. 2000 x 1000) are rejected with
+ * FGPR_EINVAL.  Calls on one device share the array: calls that overlap in
+ * time on one device (two streams, two threads) race and must be ordered by
+ * the caller.  Same results as fgpr_tv, bitwise. */
+#define FGPR_TV_STATIC_BYTES (64u << 20)
+int fgpr_tv_static(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+                   float* out, void* stream);
+
 /* out[0] = a[0] + a[s] + a[2s] + ..., out[1] the same over b (n terms,
  * stride s, FP32, in index order): the sequential sums of the first model
  * row that set the CPML sigma_max (compute.py:278-281), on the device so
