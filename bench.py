@@ -18,6 +18,7 @@ one-shot-per-launch path and a short run of the serving loop (p50 / p99).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -260,8 +261,11 @@ def c3_line(dev, args):
     value = cu / (elapsed / args.steps) / 1e9
     log(f"C3: {value:.1f} Gcell-updates/s")
     roofline, kernels = multi_roofline(ub, tb, args.profile_reps)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(wls[3], args.cpu_steps_c3)
     summ = ub.summary()
+    del ub, res, probs
+    gc.collect()                       # jobs hold their operations' closures (cycles)
+    torch.cuda.empty_cache()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wls[3], args.cpu_steps_c3)
     return {
         "metric": METRIC,
         "value": round(value, 3),
@@ -360,6 +364,8 @@ def one_at_a_time(dev):
     (autotuned variant, captured graphs, shot streams, the history kept as
     far as it fits)."""
     import torch
+    gc.collect()
+    torch.cuda.empty_cache()
     from fastgpr.workloads import c3_uneven
     wls = c3_uneven()
     obs = []
@@ -642,6 +648,8 @@ def serve_bench(dev, seconds, rate):
     import threading
     import numpy as np
     import torch
+    gc.collect()
+    torch.cuda.empty_cache()
     from fastgpr.serving import Request, Server, poisson_trace, serving_request
     rng = np.random.default_rng(4)
     arrivals = poisson_trace(rate, seconds, seed=4)

@@ -1552,36 +1552,76 @@ int launch_tb(const fgpr_problem& p, const Dev& d, bool adj, int parity, int kof
     return FGPR_OK;
 }
 
-// Captured graphs: one per (descriptor contents, direction, chunk).  The
-// descriptor holds the device pointers baked into the graph, so it is the
-// cache key byte for byte.
+// Captured graphs: one per (descriptor, direction, chunk).  The key lists
+// the descriptor's fields by name -- the device pointers baked into the
+// graph and the sizes -- never its raw bytes (the struct has padding).  The
+// cache holds at most g_graph_limit executables: past that, the least
+// recently launched one is destroyed once its last launch has finished (an
+// event recorded after every launch), so callers whose buffers move between
+// calls do not accumulate graphs.
+static_assert(sizeof(fgpr_problem) == 280, "a descriptor field was added: extend make_key");
+
 struct GraphKey {
-    std::vector<unsigned char> bytes;
-    bool operator<(const GraphKey& o) const { return bytes < o.bytes; }
+    std::vector<int64_t> v;
+    bool operator<(const GraphKey& o) const { return v < o.v; }
+};
+
+struct GraphEntry {
+    hipGraphExec_t ex = nullptr;
+    hipEvent_t done = nullptr;
+    uint64_t used = 0;
+    bool evictable = true;
 };
 
 std::mutex g_graph_mu;
-std::map<GraphKey, hipGraphExec_t> g_graphs;
+std::map<GraphKey, GraphEntry> g_graphs;
+uint64_t g_graph_clock = 0;
+size_t g_graph_limit = 64;
 
 GraphKey make_key(const fgpr_problem& p, int kind, int chunk) {
     GraphKey k;
-    k.bytes.resize(sizeof(fgpr_problem) + 2 * sizeof(int));
-    std::memcpy(k.bytes.data(), &p, sizeof(fgpr_problem));
-    std::memcpy(k.bytes.data() + sizeof(fgpr_problem), &kind, sizeof(int));
-    std::memcpy(k.bytes.data() + sizeof(fgpr_problem) + sizeof(int), &chunk, sizeof(int));
+    auto I = [&](int64_t x) { k.v.push_back(x); };
+    auto P = [&](const void* x) { k.v.push_back((int64_t)(uintptr_t)x); };
+    auto F = [&](float x) { int32_t b; std::memcpy(&b, &x, 4); k.v.push_back(b); };
+    I(p.NI); I(p.NJ); I(p.P); I(p.S); I(p.nt); I(p.nsrc); I(p.nrx); I(p.pitch);
+    F(p.dx); F(p.dt);
+    P(p.coef); P(p.pml); P(p.src); P(p.srcscale); P(p.rx); P(p.Ez); P(p.Hx); P(p.Hy); P(p.psi);
+    P(p.traces); P(p.snaps); P(p.resid); P(p.grad_eps); P(p.grad_sig); P(p.ctr);
+    P(p.Ez2); P(p.Hx2); P(p.Hy2); P(p.psi2);
+    I(p.tb_ri); I(p.tb_nt); I(p.tb_k); I(p.tb_ti); I(p.tb_tj);
+    P(p.tb_rx_ptr); P(p.tb_rx_idx); P(p.tb_inj_ptr); P(p.tb_inj_idx); P(p.gshot); P(p.scratch);
+    I(p.snap_base); I(p.nsnap); I(p.snap_mod); I(p.snap_tiled);
+    I(kind); I(chunk);
     return k;
 }
 
-// kind 0 = forward (counter advances by +chunk), 1 = reference adjoint,
-// 2 = exact adjoint (both -chunk).
-int get_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
-    std::lock_guard<std::mutex> lk(g_graph_mu);
-    GraphKey key = make_key(p, kind, chunk);
-    auto it = g_graphs.find(key);
-    if (it != g_graphs.end()) {
-        *out = it->second;
-        return FGPR_OK;
+void destroy_entry(GraphEntry& e) {
+    if (e.done) {
+        (void)hipEventSynchronize(e.done);
+        (void)hipEventDestroy(e.done);
     }
+    if (e.ex) (void)hipGraphExecDestroy(e.ex);
+    e.ex = nullptr;
+    e.done = nullptr;
+}
+
+// Least recently launched entries go first; `keep` is never evicted.
+void evict_over_limit(const GraphEntry* keep) {
+    while (g_graphs.size() > g_graph_limit) {
+        auto victim = g_graphs.end();
+        for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+            if (&it->second != keep && it->second.evictable &&
+                (victim == g_graphs.end() || it->second.used < victim->second.used))
+                victim = it;
+        if (victim == g_graphs.end()) return;
+        destroy_entry(victim->second);
+        g_graphs.erase(victim);
+    }
+}
+
+// kind 0 = forward (counter advances by +chunk), 1 = reference adjoint,
+// 2 = exact adjoint (both -chunk).  Called with g_graph_mu held.
+int capture_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
     hipStream_t cs;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     const Dev d = to_dev(p);
@@ -1612,8 +1652,37 @@ int get_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
     ec = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ec != hipSuccess) return fail(FGPR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
-    g_graphs.emplace(std::move(key), ex);
     *out = ex;
+    return FGPR_OK;
+}
+
+// Launch the graph of (p, kind, chunk) `reps` times on st, capturing it on
+// first use; records the entry's completion event after the launches.
+int launch_graph(const fgpr_problem& p, int kind, int chunk, int reps, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    GraphKey key = make_key(p, kind, chunk);
+    auto it = g_graphs.find(key);
+    if (it == g_graphs.end()) {
+        GraphEntry e;
+        int rc = capture_graph(p, kind, chunk, &e.ex);
+        if (rc) return rc;
+        if (hipEventCreateWithFlags(&e.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGraphExecDestroy(e.ex);
+            return fail(FGPR_EHIP, "hipEventCreateWithFlags failed");
+        }
+        it = g_graphs.emplace(std::move(key), e).first;
+    }
+    GraphEntry& e = it->second;
+    for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(e.ex, st));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cap));
+    if (cap == hipStreamCaptureStatusNone) {
+        HIP_TRY(hipEventRecord(e.done, st));
+    } else {
+        e.evictable = false;          // launched into the caller's capture: kept until released
+    }
+    e.used = ++g_graph_clock;
+    evict_over_limit(&e);
     return FGPR_OK;
 }
 
@@ -1901,11 +1970,10 @@ int fgpr_forward(const fgpr_problem* p, int n0, int n1, int mode, int chunk, voi
     int n = n0;
     if (tb) chunk = tb_chunk(chunk, p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n1 - n0 >= chunk) {
-        hipGraphExec_t ex;
-        rc = get_graph(q, 0, chunk, &ex);
-        if (rc) return rc;
+        const int reps = (n1 - n0) / chunk;
         hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(64), 0, st, q.ctr, n0);
-        for (; n + chunk <= n1; n += chunk) HIP_TRY(hipGraphLaunch(ex, st));
+        if ((rc = launch_graph(q, 0, chunk, reps, st))) return rc;
+        n += reps * chunk;
     }
     q.ctr = nullptr;
     const Dev d = to_dev(q);
@@ -1958,11 +2026,10 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
         fgpr_problem q = *p;
         int n = n_hi;
         if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n_hi - n_lo + 1 >= chunk) {
-            hipGraphExec_t ex;
-            rc = get_graph(q, 2, chunk, &ex);
-            if (rc) return rc;
+            const int reps = (n_hi - n_lo + 1) / chunk;
             hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(64), 0, st, q.ctr, n_hi);
-            for (; n - chunk + 1 >= n_lo; n -= chunk) HIP_TRY(hipGraphLaunch(ex, st));
+            if ((rc = launch_graph(q, 2, chunk, reps, st))) return rc;
+            n -= reps * chunk;
         }
         q.ctr = nullptr;
         const Dev d = to_dev(q);
@@ -1977,11 +2044,10 @@ int fgpr_adjoint(const fgpr_problem* p, int n_hi, int n_lo, int grad_mode, int m
     int n = n_hi;
     if (tb) chunk = tb_chunk(chunk, p->tb_k);
     if (mode == FGPR_EXEC_GRAPH && chunk > 0 && p->ctr && n_hi - n_lo + 1 >= chunk) {
-        hipGraphExec_t ex;
-        rc = get_graph(q, 1, chunk, &ex);
-        if (rc) return rc;
+        const int reps = (n_hi - n_lo + 1) / chunk;
         hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(64), 0, st, q.ctr, n_hi);
-        for (; n - chunk + 1 >= n_lo; n -= chunk) HIP_TRY(hipGraphLaunch(ex, st));
+        if ((rc = launch_graph(q, 1, chunk, reps, st))) return rc;
+        n -= reps * chunk;
     }
     q.ctr = nullptr;
     const Dev d = to_dev(q);
@@ -2284,9 +2350,23 @@ int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, f
 
 int fgpr_release_graphs(void) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : g_graphs) destroy_entry(kv.second);
     g_graphs.clear();
     return FGPR_OK;
+}
+
+int fgpr_graph_count(void) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    return (int)g_graphs.size();
+}
+
+int fgpr_set_graph_limit(int n) {
+    if (n < 1) return fail(FGPR_EINVAL, "graph cache limit must be >= 1");
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    const int prev = (int)g_graph_limit;
+    g_graph_limit = (size_t)n;
+    evict_over_limit(nullptr);
+    return prev;
 }
 
 }  // extern "C"

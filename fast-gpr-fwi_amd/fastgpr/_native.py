@@ -93,6 +93,8 @@ SIGNATURES = {
     "fgpr_tb_settle": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_void_p]),
     "fgpr_reduce_shots": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p]),
     "fgpr_release_graphs": (ctypes.c_int, []),
+    "fgpr_graph_count": (ctypes.c_int, []),
+    "fgpr_set_graph_limit": (ctypes.c_int, [ctypes.c_int]),
     "fgpr_tv_workspace_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int]),
     "fgpr_row_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_void_p]),

@@ -135,11 +135,12 @@ def history(G, checkpoint) -> History | None:
 
     ``checkpoint`` = None / False: full history.  An int C: a ring of C + 1
     slots (recompute segments of C steps).  "auto": the full history when it
-    fits in the free device memory (counting what torch's caching allocator
-    holds unused) less a reserve for the rest of the gradient (5 % of the
-    device, at least 8 GiB); otherwise the largest ring that does.  A history
-    under a quarter of the device's memory is stored without asking the
-    device for its free memory."""
+    fits in the free device memory less a reserve for the rest of the
+    gradient (5 % of the device, at least 8 GiB); otherwise the largest ring
+    that does.  A history under a quarter of the device's memory is stored
+    without asking the device; for a larger one torch's cached blocks are
+    released first (they may be too fragmented to hold it), which waits for
+    the device -- negligible beside a gradient of that size."""
     if checkpoint is None or checkpoint is False:
         return None
     nt = G.iterations
@@ -150,8 +151,8 @@ def history(G, checkpoint) -> History | None:
         total = torch.cuda.get_device_properties(G.device).total_memory
         if full <= 0.25 * total:
             return None
+        torch.cuda.empty_cache()
         free, _ = torch.cuda.mem_get_info(G.device)
-        free += torch.cuda.memory_reserved(G.device) - torch.cuda.memory_allocated(G.device)
         budget = int(free - max(8 << 30, 0.05 * total))
         if full <= budget:
             return None

@@ -330,6 +330,20 @@ def release_graphs():
     _native.check(_native.lib.fgpr_release_graphs(), "release graphs")
 
 
+def graph_count() -> int:
+    """Captured graphs the library's cache holds."""
+    return int(_native.lib.fgpr_graph_count())
+
+
+def set_graph_limit(n: int) -> int:
+    """Bound the library's graph cache (least recently launched graphs are
+    destroyed beyond it); returns the previous bound."""
+    prev = int(_native.lib.fgpr_set_graph_limit(int(n)))
+    if prev < 0:
+        _native.check(prev, "set graph limit")
+    return prev
+
+
 def record_sweeps(on: bool):
     """Start recording a pair of HIP events on the calling stream around
     every full-history sweep (forward and reference adjoint: from before the

@@ -318,11 +318,21 @@ int fgpr_snap_pack(const fgpr_problem* p, int slot, const float* Ez, void* strea
 int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, float* out,
              void* stream);
 
-/* Release every HIP graph the library captured.  Graphs are cached by the
- * descriptor's bytes (device pointers and sizes), so callers whose buffers move
- * accumulate them; call this only when none of them is still running (after
- * synchronising the device). */
+/* Release every HIP graph the library captured; call it only when none of
+ * them is still running (after synchronising the device).  Graphs are cached
+ * by the descriptor's fields (device pointers and sizes), direction and
+ * chunk; the cache holds at most fgpr_set_graph_limit() executables (64 by
+ * default) and destroys the least recently launched one beyond that, after
+ * its last launch has finished, so callers whose buffers move between calls
+ * keep a bounded number of graphs. */
 int fgpr_release_graphs(void);
+
+/* Number of captured graphs the cache holds now. */
+int fgpr_graph_count(void);
+
+/* Set the cache bound (n >= 1, evicting at once if over it); returns the
+ * previous bound. */
+int fgpr_set_graph_limit(int n);
 
 #ifdef __cplusplus
 }

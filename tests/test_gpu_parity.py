@@ -280,3 +280,33 @@ def test_pml_profile_on_device_matches_host(hip_device, P, NJ):
     dev = g.pml_profile(P, 0.025, 4.1e-11, er.to(hip_device), mr.to(hip_device))
     assert dev.device.type == "cuda" and dev.shape == host.shape
     np.testing.assert_allclose(dev.cpu().numpy(), host.numpy(), rtol=1.2e-6, atol=0)
+
+
+def test_graph_cache_stays_bounded(hip_device):
+    """With the cache bound at one graph every sweep evicts the other
+    direction's graph (after its last launch has finished) and captures its
+    own again; with buffers that move between calls the cache never exceeds
+    its bound.  Every gradient is bitwise the first one."""
+    import torch
+    from fastgpr import solver
+    from fastgpr.workloads import small_case
+    wl = small_case(NI=40, NJ=36, P=6, S=2, nt=120, nrx=4)
+    resid = np.random.default_rng(3).standard_normal((wl.S, wl.nt, wl.rx.shape[1])).astype(np.float32)
+    kw = dict(resid=resid, time_block=(64, 1024, 6), shot_streams=1, graph_chunk=12)
+    solver.release_graphs()
+    _, first = run_product(wl, hip_device, **kw)
+    prev = solver.set_graph_limit(1)
+    try:
+        assert solver.graph_count() <= 1
+        for limit, calls in ((1, 4), (3, 10)):
+            solver.set_graph_limit(limit)
+            hold = []
+            for k in range(calls):
+                hold.append(torch.empty(4096 * (k + 1), dtype=torch.uint8, device=hip_device))
+                _, g = run_product(wl, hip_device, **kw)
+                assert solver.graph_count() <= limit, (limit, k)
+                for n in ("er", "se"):
+                    assert np.array_equal(g[n], first[n]), (limit, k, n)
+    finally:
+        solver.set_graph_limit(prev)
+        solver.release_graphs()
