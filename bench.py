@@ -75,6 +75,12 @@ def parse(argv=None):
                     help="only run the serving loop under Poisson arrivals for this many "
                          "seconds of arrivals; prints its own JSON line")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve arrival rate (requests/s)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then "
+                         "WRITE_SIZE; CSV output) into profiles/traffic.json; uses no GPU")
+    ap.add_argument("--kernel-stats", metavar="DB",
+                    help="per (kernel, grid size) dispatch counts and durations of a rocprofv3 "
+                         "kernel-trace database; uses no GPU")
     ap.add_argument("--kernel-busy", nargs=3, metavar=("DB", "T0_NS", "T1_NS"),
                     help="read a rocprofv3 kernel-trace database and print how much of the "
                          "monotonic window [T0, T1] (the serving line's window_ns) had a kernel "
@@ -114,6 +120,29 @@ def kernel_bytes(kernel: int, S: int, NI: int, NJ: int, P: int, nrx: int, k: int
     raise ValueError(kernel)
 
 
+def kernel_stats(db: str, top: int = 12) -> list:
+    """Dispatch count, total / average / min / max duration (us) per
+    (kernel, grid size) of a rocprofv3 kernel-trace database, largest total
+    first."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    gx = next((c for c in ("grid_size", "grid_size_x", "grid_x") if c in cols), None)
+    q = f"select name, start, end{', ' + gx if gx else ''} from kernels"
+    acc = {}
+    for row in con.execute(q):
+        name = re.sub(r"\(.*", "", row[0].replace("(anonymous namespace)::", "")).replace("void ", "")
+        key = (name, int(row[3]) if gx else 0)
+        acc.setdefault(key, []).append((row[2] - row[1]) / 1e3)
+    out = []
+    for (name, grid), d in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        out.append({"kernel": name, "grid": grid, "calls": len(d), "total_ms": round(sum(d) / 1e3, 3),
+                    "avg_us": round(sum(d) / len(d), 2), "min_us": round(min(d), 2),
+                    "max_us": round(max(d), 2)})
+    return out
+
+
 def kernel_busy(db: str, t0: int, t1: int, top: int = 10) -> dict:
     """Union of the kernel intervals of a rocprofv3 trace inside [t0, t1]
     (CLOCK_MONOTONIC ns, the clock of rocprofv3 timestamps and of
@@ -149,6 +178,12 @@ def kernel_busy(db: str, t0: int, t1: int, top: int = 10) -> dict:
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
+    if args.kernel_stats:
+        print(json.dumps(kernel_stats(args.kernel_stats), indent=1))
+        return
     if args.kernel_busy:
         db, t0, t1 = args.kernel_busy
         print(json.dumps(kernel_busy(db, int(t0), int(t1))))
@@ -312,7 +347,9 @@ def multi_roofline(ub, tb, reps):
     sp = ctypes.c_void_p(st.cuda_stream)
     per = {}
     K = tb[2]
-    for adj, prob, n in ((False, job.pf_plain, 0), (True, job.pa, G.iterations - 1)):
+    # the forward as the sweep runs it, with its snapshot stores
+    fwd = job.pf if job.gradient else job.pf_plain
+    for adj, prob, n in ((False, fwd, G.iterations - K), (True, job.pa, G.iterations - 1)):
         item = _native.TBItem()
         item.prob = ctypes.pointer(prob)
         item.n0, item.nsteps, item.parity = n, K, 0
@@ -566,6 +603,70 @@ def measured_traffic(path, key):
     return None if k is None else k.get("bytes_per_launch")
 
 
+# Dispatches the counter summary attributes to a workload: kernel name
+# prefix and grid size (work-items) of the probe's launches.
+PMC_KERNELS = {
+    "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,false>":
+        ("k_tb_multi<64, 1024, 6, false>", 15 * 24 * 12 * 1024),
+    "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,true>":
+        ("k_tb_multi<64, 1024, 6, true>", 15 * 24 * 12 * 1024),
+    "c2_crosshole_400x200_13shots|k_tb<64,1024,6,false>": ("k_tb<64, 1024, 6, false>", 13 * 32 * 1024),
+    "c2_crosshole_400x200_13shots|k_tb<64,1024,6,true>": ("k_tb<64, 1024, 6, true>", 13 * 32 * 1024),
+}
+COPY_BYTES = 3 * 256 * 1024 * 1024       # the probe's three 256 MiB copies, read and written
+
+
+def _pmc_rows(root):
+    """(kernel name, grid size, counter, value) of every dispatch in the
+    counter_collection CSV files under ``root``."""
+    import csv
+    out = []
+    for f in Path(root).rglob("*counter_collection.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                name = r["Kernel_Name"].replace("(anonymous namespace)::", "")
+                name = name[5:] if name.startswith("void ") else name
+                out.append((name, int(r["Grid_Size"]), r["Counter_Name"], float(r["Counter_Value"]),
+                            int(r.get("Dispatch_Id") or 0)))
+    return sorted(out, key=lambda t: t[4])
+
+
+def pmc_summary(fetch_dir, write_dir, out=None):
+    """HBM bytes per launch of the dominant kernels from two counter runs of
+    the probe: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
+    what the same counter reads for the probe's device copies of known size
+    (3 x 256 MiB read and written)."""
+    res = {"calibration": {}, "measured": {}}
+    per = {}
+    for which, root in (("fetch", fetch_dir), ("write", write_dir)):
+        rows = _pmc_rows(root)
+        # the calibration copies are the copy dispatches before the first solver kernel
+        first = next((i for i, r in enumerate(rows) if r[0].startswith("k_")), len(rows))
+        copy = [v for n, g, c, v, _ in rows[:first] if "copyBuffer" in n]
+        if not copy:
+            raise SystemExit(f"no copy-kernel dispatches under {root}")
+        kib = sum(copy)
+        res["calibration"][which] = {"copy_dispatches": len(copy), "kib_total": kib,
+                                     "factor": COPY_BYTES / (kib * 1024)}
+        for key, (prefix, grid) in PMC_KERNELS.items():
+            vals = [v for n, g, c, v, _ in rows if n.startswith(prefix) and g == grid]
+            if vals:
+                per.setdefault(key, {})[which] = (len(vals), sum(vals) / len(vals))
+    for key, d in per.items():
+        if "fetch" not in d or "write" not in d:
+            continue
+        rd = d["fetch"][1] * 1024 * res["calibration"]["fetch"]["factor"]
+        wr = d["write"][1] * 1024 * res["calibration"]["write"]["factor"]
+        res["measured"][key] = {"dispatches": [d["fetch"][0], d["write"][0]],
+                                "fetch_kib_per_dispatch": round(d["fetch"][1], 1),
+                                "write_kib_per_dispatch": round(d["write"][1], 1),
+                                "read_bytes": int(rd), "write_bytes": int(wr),
+                                "bytes_per_launch": int(rd + wr)}
+    with open(out or TRAFFIC_FILE, "w") as f:
+        json.dump(res, f, indent=1)
+    return res
+
+
 def pmc_probe(dev, args):
     """Program for the FETCH_SIZE / WRITE_SIZE counter runs: a 256 MiB
     device copy (the calibration: 256 MiB read + 256 MiB written), then the
@@ -708,12 +809,13 @@ def serve_bench(dev, seconds, rate):
     worst = 0.0
     done = [r for r in reqs if r.result and "er" in r.result]
     picks = [r for r in done if r.rid in sampled] if lazy else done[:: max(1, len(done) // 8)][:8]
-    budget = srv.budget
     del srv
+    gc.collect()
     torch.cuda.empty_cache()
-    picks = picks if picks else []
     for r in picks:
-        a = Server(dev, hbm_budget=budget)
+        gc.collect()                     # release the last arena before sizing the next
+        torch.cuda.empty_cache()
+        a = Server(dev)
         ra = a.submit(Request(r.rid, r.wl, r.model, r.obs, tb=r.tb))
         a.run()
         torch.cuda.synchronize()
@@ -738,6 +840,7 @@ def serve_bench(dev, seconds, rate):
         "recompute_share": round(summ["recompute_share"], 4),
         "launches": summ["launches"], "ticks": summ["ticks"],
         "host_ms_per_tick": round(summ["host_ms_per_tick"], 3),
+        "admit_host_s": round(summ["admit_host_s"], 3),
         "window_ns": [m0, m1],
         "alone_vs_mix_max_rel_diff": worst}
 

@@ -135,12 +135,13 @@ def history(G, checkpoint) -> History | None:
 
     ``checkpoint`` = None / False: full history.  An int C: a ring of C + 1
     slots (recompute segments of C steps).  "auto": the full history when it
-    fits in the free device memory less a reserve for the rest of the
-    gradient (5 % of the device, at least 8 GiB); otherwise the largest ring
-    that does.  A history under a quarter of the device's memory is stored
-    without asking the device; for a larger one torch's cached blocks are
-    released first (they may be too fragmented to hold it), which waits for
-    the device -- negligible beside a gradient of that size."""
+    fits in the free device memory (counting what torch's caching allocator
+    holds unused) less a reserve for the rest of the gradient (5 % of the
+    device, at least 8 GiB); otherwise the largest ring that does.  A history
+    under a quarter of the device's memory is stored without asking the
+    device for its free memory.  (Should the cached memory be too fragmented
+    for the buffer, ``solver.device_empty`` releases it and tries once
+    more.)"""
     if checkpoint is None or checkpoint is False:
         return None
     nt = G.iterations
@@ -151,8 +152,8 @@ def history(G, checkpoint) -> History | None:
         total = torch.cuda.get_device_properties(G.device).total_memory
         if full <= 0.25 * total:
             return None
-        torch.cuda.empty_cache()
         free, _ = torch.cuda.mem_get_info(G.device)
+        free += torch.cuda.memory_reserved(G.device) - torch.cuda.memory_allocated(G.device)
         budget = int(free - max(8 << 30, 0.05 * total))
         if full <= budget:
             return None
@@ -235,7 +236,7 @@ def forward(G, er, se, mr, hist: History):
     b = _s.Batch(G, store=False, tb=G.time_block)
     scale = _s.emitting_sources(G, b)
     st = States(b, hist.saves())
-    ring = torch.empty((hist.R,) + b.snap_slot_shape(), dtype=torch.float32, device=dev)
+    ring = _s.device_empty((hist.R,) + b.snap_slot_shape(), torch.float32, dev)
     plain = b.problem(coef, srcscale=scale)
     kept = _ring_problem(b, coef, scale, ring, hist.R)
     for n0, n1, save, store in forward_cuts(hist):

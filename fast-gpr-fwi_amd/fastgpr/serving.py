@@ -64,6 +64,8 @@ class Request:
     tb: tuple | None = None
     history: History | None = None
     recompute_steps: int = 0
+    _sizes: tuple | None = None        # (variant, smallest ring, its bytes, whole-history bytes)
+    _held_at: int = -1                 # arena version of the last admission that did not fit
 
 
 @dataclass
@@ -79,6 +81,7 @@ class Stats:
     max_active: int = 0
     ticks: int = 0
     tick_host_s: float = 0.0        # host time spent packing and launching ticks
+    admit_host_s: float = 0.0       # host time spent admitting (planning, building jobs)
     holds: int = 0                  # admissions put off because no free block held the request
     refused: int = 0                # requests that failed (invalid, too large, non-finite)
     cancelled: int = 0
@@ -95,6 +98,7 @@ class Stats:
                     self.cell_updates + self.recompute_cell_updates, 1),
                 "wall_s": wall, "launches": self.launches, "ticks": self.ticks,
                 "host_ms_per_tick": 1e3 * self.tick_host_s / max(self.ticks, 1),
+                "admit_host_s": self.admit_host_s,
                 "holds": self.holds, "refused": self.refused, "cancelled": self.cancelled,
                 "partial_histories": self.partial_histories,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
@@ -130,6 +134,7 @@ class Server:
         self._tick_events: list = []
         self._cancel: set = set()
         self._lock = threading.Lock()
+        self._arena_version = 0         # advances whenever a block is released
 
     # --- client side -----------------------------------------------------------
     def submit(self, req: Request) -> Request:
@@ -164,9 +169,11 @@ class Server:
         if req.obs is None:
             nbytes = batching.job_bytes(*spec, wl.nt, False, tb)
             return tb, None, nbytes, nbytes
-        low = _smallest_history(spec, tb)
-        smallest = batching.job_bytes(*spec, low, True, tb)
-        full = batching.job_bytes(*spec, wl.nt, True, tb)
+        if req._sizes is None or req._sizes[0] != tb:      # the request's own sizes, once
+            low = _smallest_history(spec, tb)
+            req._sizes = (tb, low, batching.job_bytes(*spec, low, True, tb),
+                          batching.job_bytes(*spec, wl.nt, True, tb))
+        _, low, smallest, full = req._sizes
         largest = self.arena.largest_free()
         share = largest if not self.active else largest // 2
         if full <= share:
@@ -195,6 +202,11 @@ class Server:
             if blocked:
                 still.append(req)
                 continue
+            if req._held_at == self._arena_version:
+                still.append(req)          # nothing was released since it did not fit
+                if now - req.arrival > self.max_hold_s:
+                    blocked = True
+                continue
             try:
                 tb, hist, nbytes, smallest = self._plan(req)
             except Exception as e:                       # invalid request: it fails alone
@@ -206,6 +218,7 @@ class Server:
             off = self.arena.alloc(nbytes)
             if off is None:
                 self.stats.holds += 1
+                req._held_at = self._arena_version
                 still.append(req)
                 if now - req.arrival > self.max_hold_s:
                     blocked = True
@@ -214,6 +227,7 @@ class Server:
                 job = self._make_job(req, tb, hist, self.arena.region(off))
             except Exception as e:
                 self.arena.release(off)
+                self._arena_version += 1
                 self._fail(req, f"could not build the job: {type(e).__name__}: {e}")
                 continue
             req.tb, req.history = tb, hist
@@ -257,6 +271,7 @@ class Server:
         for req, job, off, nbytes in self.active:
             if job.done and job.done_event is not None and job.done_event.query():
                 self.arena.release(off)
+                self._arena_version += 1
                 self.in_use -= nbytes
                 self.sched.jobs.remove(job)
                 req.done_at = time.perf_counter()
@@ -283,7 +298,9 @@ class Server:
         """Admit, advance one tick, reap.  Returns False when idle."""
         if not self.stats.started:
             self.stats.started = time.perf_counter()
+        t = time.perf_counter()
         self._admit()
+        self.stats.admit_host_s += time.perf_counter() - t
         busy = bool(self.sched.active())
         if busy:
             before = self.sched.launches

@@ -36,6 +36,16 @@ def to_device(t: torch.Tensor, device) -> torch.Tensor:
     return t.contiguous().pin_memory().to(device, non_blocking=True)
 
 
+def device_empty(shape, dtype, device) -> torch.Tensor:
+    """torch.empty that, when the caching allocator's free blocks are too
+    fragmented for a large buffer, releases them and tries once more."""
+    try:
+        return torch.empty(shape, dtype=dtype, device=device)
+    except torch.OutOfMemoryError:
+        torch.cuda.empty_cache()
+        return torch.empty(shape, dtype=dtype, device=device)
+
+
 def require_hip(device) -> torch.device:
     device = torch.device(device)
     if device.type != "cuda":
@@ -165,7 +175,7 @@ class Batch:
             as_dev = lambda a: to_device(torch.from_numpy(np.ascontiguousarray(a)), dev)
             self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
                               as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
-        empty = ((lambda shape, dt: torch.empty(shape, dtype=dt, device=dev)) if alloc is None
+        empty = ((lambda shape, dt: device_empty(shape, dt, dev)) if alloc is None
                  else alloc)
         self.snaps = empty((self.nt,) + self.snap_slot_shape(), torch.float32) if store else None
 

@@ -67,3 +67,36 @@ def test_kernel_busy_takes_the_union_inside_the_window(tmp_path):
     assert abs(r["busy_s"] - 200e-9) < 1e-15 and abs(r["busy_fraction"] - 200 / 900) < 1e-12
     assert r["kernels"]["k_a"] == {"launches": 2, "total_s": round(150e-9, 4), "avg_us": 0.07}
     assert r["kernels"]["k_b"]["launches"] == 1
+
+
+def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
+    """Counter KiB per dispatch of the probe's kernels, scaled by what the
+    same counter read for the copies of known size, keyed by workload; other
+    grid sizes of the same kernel are not attributed."""
+    import bench
+    head = ("Correlation_Id,Dispatch_Id,Agent_Id,Queue_Id,Process_Id,Thread_Id,Grid_Size,"
+            "Kernel_Id,Kernel_Name,Workgroup_Size,LDS_Block_Size,Scratch_Size,VGPR_Count,"
+            "Accum_VGPR_Count,SGPR_Count,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp\n")
+    g3 = 15 * 24 * 12 * 1024
+    copy_kib = bench.COPY_BYTES / 1024 / 2            # counter reads half the true bytes
+
+    def write(d, counter, kv):
+        d.mkdir()
+        rows = [f'{i},{i},1,1,1,1,{g},0,"{n}",256,0,0,8,0,16,{counter},{v},0,1\n'
+                for i, (n, g, v) in enumerate(kv)]
+        (d / "1_counter_collection.csv").write_text(head + "".join(rows))
+    write(tmp_path / "f", "FETCH_SIZE",
+          [("__amd_rocclr_copyBuffer", 1024, copy_kib / 3)] * 3 +
+          [("void (anonymous namespace)::k_tb_multi<64, 1024, 6, true>(unsigned char const*, int)", g3, 1000.0)] * 2 +
+          [("void (anonymous namespace)::k_tb_multi<64, 1024, 6, true>(unsigned char const*, int)", 999, 5e9)] +
+          [("__amd_rocclr_copyBuffer", 1024, 7e9)])       # a later copy: not calibration
+    write(tmp_path / "w", "WRITE_SIZE",
+          [("__amd_rocclr_copyBuffer", 1024, copy_kib / 3)] * 3 +
+          [("(anonymous namespace)::k_tb_multi<64, 1024, 6, true>(unsigned char const*, int)", g3, 500.0)] * 2)
+    res = bench.pmc_summary(tmp_path / "f", tmp_path / "w", out=tmp_path / "t.json")
+    m = res["measured"]["c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,true>"]
+    assert abs(res["calibration"]["fetch"]["factor"] - 2.0) < 1e-12
+    assert m["read_bytes"] == 2 * 1000 * 1024 and m["write_bytes"] == 2 * 500 * 1024
+    assert m["bytes_per_launch"] == 3 * 1000 * 1024
+    assert bench.measured_traffic(tmp_path / "t.json",
+                                  "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,true>") == 3072000

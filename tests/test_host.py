@@ -122,9 +122,9 @@ def test_shot_groups_are_held_to_spare_hardware_queues(monkeypatch):
     assert solver.shot_groups(G(shot_streams=5)) == 5
 
 
-def test_auto_checkpoint_releases_the_cache_and_keeps_a_reserve(monkeypatch):
+def test_auto_checkpoint_counts_cached_memory_and_keeps_a_reserve(monkeypatch):
     """checkpoint="auto" stores the history when it fits in the free memory
-    (after releasing torch's cached blocks) less the reserve; otherwise it
+    plus what torch's allocator caches unused, less the reserve; otherwise it
     keeps the largest snapshot ring that fits (device queries stubbed: no GPU
     here)."""
     from types import SimpleNamespace
@@ -133,17 +133,16 @@ def test_auto_checkpoint_releases_the_cache_and_keeps_a_reserve(monkeypatch):
     GiB = 1 << 30
     monkeypatch.setattr(torch.cuda, "get_device_properties",
                         lambda d: SimpleNamespace(total_memory=288 * GiB))
-    cache = {"held": 140 * GiB}
-    monkeypatch.setattr(torch.cuda, "empty_cache", lambda: cache.update(held=0, free=cache["free"] + cache["held"]))
-    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (cache["free"], 288 * GiB))
-    cache["free"] = 100 * GiB
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda d: (100 * GiB, 288 * GiB))
+    monkeypatch.setattr(torch.cuda, "memory_allocated", lambda d: 10 * GiB)
     G = SimpleNamespace(iterations=1500, S=13, NI=1600, NJ=800, time_block=None, options={},
                         graph_chunk=32, device="cuda:0", pmlthick=20)
     slot = 13 * 1600 * _native.pitch(800) * 4
     full = 1500 * slot
     assert 72 * GiB < full < 100 * GiB
-    assert ck.history(G, "auto") is None                  # 100 + 140 released - 14.4 reserve
-    cache.update(free=100 * GiB, held=0)
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 150 * GiB)
+    assert ck.history(G, "auto") is None                  # 100 + 140 cached - 14.4 reserve
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda d: 10 * GiB)
     h = ck.history(G, "auto")                              # 100 - 14.4 < full
     budget = int(100 * GiB - 0.05 * 288 * GiB)
     assert h is not None and not h.full
