@@ -46,8 +46,9 @@ def parse(argv=None):
                     help="c3: the uneven batch (default line); c2: the cross-hole batch alone")
     ap.add_argument("--quick", action="store_true",
                     help="the C3 line only (no one-at-a-time, C2, plain-path or serving keys)")
-    ap.add_argument("--c3-variant", default="64,1024,6",
-                    help="time-blocking variant of the uneven batch (rows,threads,steps)")
+    ap.add_argument("--c3-variant", default="auto",
+                    help="time-blocking variant of the uneven batch's jobs: 'auto' (the autotuner's "
+                         "pick per model) or rows,threads,steps for all")
     ap.add_argument("--exec", dest="exec_mode", default="graph", choices=["graph", "eager"])
     ap.add_argument("--chunk", type=int, default=None,
                     help="C2: steps per captured graph (default: the library's choice)")
@@ -260,7 +261,8 @@ def c3_problems(dev, tbs):
     probs = []
     for wl, tb in zip(wls, tbs):
         with torch.no_grad():
-            obs = _compute(wl, dev, *_tensors(wl, dev, (wl.er, wl.se)), time_block=tb).detach()
+            obs = _compute(wl, dev, *_tensors(wl, dev, (wl.er, wl.se)),
+                           time_block=(64, 1024, 6) if tb == "auto" else tb).detach()
         er, se, mr = _tensors(wl, dev, (wl.er0, wl.se0))
         er, se = torch.clamp(er, min=1), torch.clamp(se, min=0)
         G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
@@ -275,12 +277,13 @@ def c3_line(dev, args):
     """The headline: the uneven batch from one captured graph."""
     import torch
     from fastgpr.uneven import UnevenBatch
-    tb = tuple(int(v) for v in args.c3_variant.split(","))
+    tb = "auto" if args.c3_variant == "auto" else tuple(int(v) for v in args.c3_variant.split(","))
     log("C3: observed traces and problems")
     wls, probs = c3_problems(dev, [tb] * 4)
     t0 = time.perf_counter()
     ub = UnevenBatch(dev, probs)
     build_s = time.perf_counter() - t0
+    tbs = [p.tb for p in ub.problems]
     log(f"C3: built and captured in {build_s:.1f} s: {ub.summary()}")
     for _ in range(args.warmup):
         ub.run()
@@ -295,7 +298,7 @@ def c3_line(dev, args):
     cu = ub.cell_updates()
     value = cu / (elapsed / args.steps) / 1e9
     log(f"C3: {value:.1f} Gcell-updates/s")
-    roofline, kernels = multi_roofline(ub, tb, args.profile_reps)
+    roofline, kernels = multi_roofline(ub, args.profile_reps)
     summ = ub.summary()
     del ub, res, probs
     gc.collect()                       # jobs hold their operations' closures (cycles)
@@ -317,7 +320,7 @@ def c3_line(dev, args):
         "config": {"workload": "c3_uneven: 4 models, 64 shots: " +
                                ", ".join(f"{w.NI}x{w.NJ}/{w.nt} steps/{w.S} shots" for w in wls),
                    "dx_m": wls[0].dx, "cpml": wls[0].P,
-                   "cell_updates_per_step": cu, "time_block": list(tb),
+                   "cell_updates_per_step": cu, "time_block": [list(t) for t in tbs],
                    "exec": "one captured HIP graph per step (all ticks, all waves)",
                    "waves": summ["waves"], "snapshot_rings": summ["rings"],
                    "recompute_steps": summ["recompute_steps"],
@@ -329,7 +332,7 @@ def c3_line(dev, args):
     }
 
 
-def multi_roofline(ub, tb, reps):
+def multi_roofline(ub, reps):
     """Average launch duration of the multi-problem kernels on the largest
     model's job (one item per launch, as in its wave), timed with HIP events
     on the launch stream, against the algorithmic bytes of a block and, when
@@ -341,6 +344,7 @@ def multi_roofline(ub, tb, reps):
     from fastgpr._native import check, lib
     k = max(range(len(ub.jobs)), key=lambda i: ub.jobs[i].cell_updates())
     job = ub.jobs[k]
+    tb = job.tb
     G = job.G
     name = f"c3_model{k}_{G.NI}x{G.NJ}_{G.S}shots"
     st = torch.cuda.current_stream()
@@ -604,15 +608,23 @@ def measured_traffic(path, key):
 
 
 # Dispatches the counter summary attributes to a workload: kernel name
-# prefix and grid size (work-items) of the probe's launches.
-PMC_KERNELS = {
-    "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,false>":
-        ("k_tb_multi<64, 1024, 6, false>", 15 * 24 * 12 * 1024),
-    "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,true>":
-        ("k_tb_multi<64, 1024, 6, true>", 15 * 24 * 12 * 1024),
-    "c2_crosshole_400x200_13shots|k_tb<64,1024,6,false>": ("k_tb<64, 1024, 6, false>", 13 * 32 * 1024),
-    "c2_crosshole_400x200_13shots|k_tb<64,1024,6,true>": ("k_tb<64, 1024, 6, true>", 13 * 32 * 1024),
-}
+# prefix and grid size (work-items) of the probe's launches, for every
+# variant the probe may run.
+def _pmc_targets() -> dict:
+    from fastgpr import _native
+    from fastgpr.batching import MULTI_VARIANTS
+    out = {}
+    for name, (NI, NJ, S), kern, variants in (
+            ("c3_model3_1200x600_15shots", (1200, 600, 15), "k_tb_multi", MULTI_VARIANTS),
+            ("c2_crosshole_400x200_13shots", (400, 200, 13), "k_tb", [(64, 1024, 6)])):
+        for ri, nt, k in variants:
+            _, _, ti, tj = _native.tb_tiles(NI, NJ, ri, nt, k)
+            for adj in ("false", "true"):
+                out[(f"{kern}<{ri}, {nt}, {k}, {adj}>", S * ti * tj * nt)] = \
+                    f"{name}|{kern}<{ri},{nt},{k},{adj}>"
+    return out
+
+
 COPY_BYTES = 3 * 256 * 1024 * 1024       # the probe's three 256 MiB copies, read and written
 
 
@@ -648,8 +660,8 @@ def pmc_summary(fetch_dir, write_dir, out=None):
         kib = sum(copy)
         res["calibration"][which] = {"copy_dispatches": len(copy), "kib_total": kib,
                                      "factor": COPY_BYTES / (kib * 1024)}
-        for key, (prefix, grid) in PMC_KERNELS.items():
-            vals = [v for n, g, c, v, _ in rows if n.startswith(prefix) and g == grid]
+        for (prefix, grid), key in _pmc_targets().items():
+            vals = [v for n, g, c, v, _ in rows if n.startswith(prefix + "(") and g == grid]
             if vals:
                 per.setdefault(key, {})[which] = (len(vals), sum(vals) / len(vals))
     for key, d in per.items():
@@ -681,12 +693,12 @@ def pmc_probe(dev, args):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    tb = tuple(int(v) for v in args.c3_variant.split(","))
+    tb = "auto" if args.c3_variant == "auto" else tuple(int(v) for v in args.c3_variant.split(","))
     wls, probs = c3_problems(dev, [tb] * 4)
     ub = UnevenBatch(dev, probs)
     ub.run()
     torch.cuda.synchronize()
-    roof, _ = multi_roofline(ub, tb, 5)
+    roof, _ = multi_roofline(ub, 5)
     torch.cuda.synchronize()
     del ub, probs
     torch.cuda.empty_cache()

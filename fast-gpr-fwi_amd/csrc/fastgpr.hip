@@ -376,15 +376,31 @@ k_e_phase(Dev d, int koff, int dir) {
 // Adjoint source: lambda_Ez[s, rx] -= resid[s, n, r] (back.cu:24-37).  A
 // receiver on i = NI or j = NJ is skipped: the E update zeroes that cell
 // before anything reads it.
+// Receivers on one cell are applied by the thread of the first of them, one
+// after another in list order (Ez - g_a - g_b ...), as the time-blocked
+// kernels do: the result does not depend on thread timing (the reference's
+// back.cu:33-34 races on such cells).
+__device__ __forceinline__ bool first_on_cell(const Dev& d, int s, int r, int ri, int rj) {
+    const int* rs = d.rx + 2 * (size_t)s * d.nrx;
+    for (int q = 0; q < r; ++q)
+        if (rs[2 * q] == ri && rs[2 * q + 1] == rj) return false;
+    return true;
+}
+
 __global__ void k_adj_inject(Dev d, int koff, int dir) {
     const int n = step_index(d, koff, dir);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.S * d.nrx) return;
     const int s = t / d.nrx, r = t - s * d.nrx;
     const int ri = d.rx[2 * t], rj = d.rx[2 * t + 1];
-    if (ri < 0 || ri >= d.NI || rj >= d.NJ) return;
-    const float g = d.resid[((size_t)s * d.nt + n) * d.nrx + r];
-    atomicAdd(&d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj], -g);
+    if (ri < 0 || ri >= d.NI || rj >= d.NJ || !first_on_cell(d, s, r, ri, rj)) return;
+    const int* rs = d.rx + 2 * (size_t)s * d.nrx;
+    const float* g = d.resid + ((size_t)s * d.nt + n) * d.nrx;
+    float* e = &d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj];
+    float v = *e;
+    for (int q = r; q < d.nrx; ++q)
+        if (rs[2 * q] == ri && rs[2 * q + 1] == rj) v = v - g[q];
+    *e = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -546,9 +562,14 @@ __global__ void k_xadj_record(Dev d, int koff, int dir) {
     if (t >= d.S * d.nrx) return;
     const int s = t / d.nrx, r = t - s * d.nrx;
     const int ri = d.rx[2 * t], rj = d.rx[2 * t + 1];
-    if (ri < 0 || ri >= d.NI || rj >= d.NJ) return;
-    atomicAdd(&d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj],
-              d.resid[((size_t)s * d.nt + n) * d.nrx + r]);
+    if (ri < 0 || ri >= d.NI || rj >= d.NJ || !first_on_cell(d, s, r, ri, rj)) return;
+    const int* rs = d.rx + 2 * (size_t)s * d.nrx;
+    const float* g = d.resid + ((size_t)s * d.nt + n) * d.nrx;
+    float* e = &d.Ez[((size_t)s * (d.NI + 1) + ri) * d.pitch + rj];
+    float v = *e;
+    for (int q = r; q < d.nrx; ++q)      // receivers on one cell in list order
+        if (rs[2 * q] == ri && rs[2 * q + 1] == rj) v = v + g[q];
+    *e = v;
 }
 
 __global__ void k_advance_ctr(int* ctr, int delta) {
@@ -1457,6 +1478,7 @@ using TBMultiKernel = void (*)(const unsigned char*, int);
 TBMultiKernel tb_multi_kernel(int ri, int nt, int k, bool adj) {
     FGPR_TBM_CASE(32, 512, 2) FGPR_TBM_CASE(32, 512, 4) FGPR_TBM_CASE(32, 512, 6)
     FGPR_TBM_CASE(64, 1024, 2) FGPR_TBM_CASE(64, 1024, 4) FGPR_TBM_CASE(64, 1024, 6)
+    FGPR_TBM_CASE(64, 1024, 8)
     FGPR_TBM_CASE(64, 512, 2) FGPR_TBM_CASE(64, 512, 4) FGPR_TBM_CASE(64, 512, 6)
     FGPR_TBM_CASE(64, 512, 8)
     return nullptr;

@@ -3,8 +3,9 @@
 For one grid (NI, NJ, P) and batch size S it times every compiled variant
 (region rows x threads x steps per launch): repeated forward and adjoint
 launches of one block on scratch buffers, split into the shot groups the
-sweep will use, and keeps the fastest per step.  Results are cached per
-process and, when FASTGPR_TUNE_CACHE names a file, in that JSON file.
+sweep will use, replayed from a captured HIP graph as the sweeps replay
+theirs, and keeps the fastest per step.  Results are cached per process and,
+when FASTGPR_TUNE_CACHE names a file, in that JSON file (written atomically).
 """
 from __future__ import annotations
 
@@ -28,10 +29,11 @@ CANDIDATES = ([(32, 256, k) for k in (1, 2, 3, 4, 6)]
 _cache: dict = {}
 
 
-def _key(G) -> str:
+def _key(G, candidates=None) -> str:
     name = torch.cuda.get_device_name(G.device)
     groups = max(1, min(int(G.options.get("shot_streams", 3) or 1), G.S, _solver.max_shot_groups()))
-    return f"{name}|{G.NI}x{G.NJ}|P{G.pmlthick}|S{G.S}|rx{G.nrx}|g{groups}"
+    cand = "all" if candidates is None else ";".join(",".join(map(str, c)) for c in sorted(candidates))
+    return f"{name}|{G.NI}x{G.NJ}|P{G.pmlthick}|S{G.S}|rx{G.nrx}|g{groups}|{cand}"
 
 
 def _load_file():
@@ -45,8 +47,10 @@ def _load_file():
 def _save_file():
     path = os.environ.get("FASTGPR_TUNE_CACHE")
     if path:
-        with open(path, "w") as f:
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
             json.dump({k: list(v) for k, v in _cache.items()}, f, indent=1)
+        os.replace(tmp, path)                 # readers never see a partial file
 
 
 def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
@@ -66,8 +70,10 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
     (``timing_reps``)."""
     K = int(tb[2])
     nt = 4 * K
-    g = dataclasses.replace(G, iterations=nt, srcwaves=G.srcwaves[:, :nt].contiguous(),
-                            time_block=tb)
+    waves = torch.zeros((G.srcwaves.shape[0], nt), dtype=G.srcwaves.dtype)
+    m = min(nt, G.srcwaves.shape[1])
+    waves[:, :m] = G.srcwaves[:, :m]          # zero past the grid's own steps: nt columns always
+    g = dataclasses.replace(G, iterations=nt, srcwaves=waves, time_block=tb)
     dev = g.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)
     coef = _solver.cell_coeffs(er, se_f, mr, g.dt32, g.dx32, False, dev)
@@ -92,17 +98,31 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
         def run(b, st, r):
             _native.check(lib.fgpr_profile_kernel(ctypes.byref(probs[groups.index(b)]), kid, step,
                                                   r, st), "profile")
-        if n > 1:
-            _solver._fan_out(dev, groups, lambda b, st: run(b, st, 2))
-        else:
-            run(groups[0], ctypes.c_void_p(main.cuda_stream), 2)
+
+        def launches(r):
+            if n > 1:
+                _solver._fan_out(dev, groups, lambda b, st: run(b, st, r))
+            else:
+                run(groups[0], ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), r)
+        launches(2)
+        torch.cuda.synchronize(dev)
+        # the sweeps replay captured graphs, so the variants are timed that
+        # way too (short launches run closer together than eager ones)
+        graph = torch.cuda.CUDAGraph()
+        try:
+            with torch.cuda.graph(graph):
+                launches(reps)
+        except RuntimeError:
+            graph = None
+        if graph is not None:
+            graph.replay()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(main)
-        if n > 1:
-            _solver._fan_out(dev, groups, lambda b, st: run(b, st, reps))
+        if graph is not None:
+            graph.replay()
         else:
-            run(groups[0], ctypes.c_void_p(main.cuda_stream), reps)
+            launches(reps)
         e1.record(main)
         e1.synchronize()
         total += e0.elapsed_time(e1) / 1e3 / reps
@@ -124,17 +144,21 @@ def timing_reps(G) -> int:
 def choose(G, er, se, mr, candidates=None) -> tuple:
     """Fastest time-blocked variant for this grid and batch (cached)."""
     _load_file()
-    key = _key(G)
+    key = _key(G, candidates)
     if key in _cache:
         return _cache[key]
-    results = {}
+    results, errors = {}, []
     reps = timing_reps(G)
     for tb in candidates or CANDIDATES:
         try:
             _native.tb_tiles(G.NI, G.NJ, *tb)
             results[tb] = time_variant(G, er, se, mr, tb, reps=reps)
-        except (RuntimeError, ValueError):
+        except (RuntimeError, ValueError) as e:
+            errors.append(f"{tb}: {e}")
             continue
+    if not results:
+        raise RuntimeError(f"no time-blocked variant runs on a {G.NI} x {G.NJ} grid with "
+                           f"{G.S} shots: " + "; ".join(errors[:4]))
     best = min(results, key=results.get)
     _cache[key] = best
     _save_file()

@@ -38,7 +38,8 @@ from .checkpoint import History, States, plan
 from ._native import check, lib
 
 MULTI_VARIANTS = [(32, 512, 2), (32, 512, 4), (32, 512, 6), (64, 1024, 2), (64, 1024, 4),
-                  (64, 1024, 6), (64, 512, 2), (64, 512, 4), (64, 512, 6), (64, 512, 8)]
+                  (64, 1024, 6), (64, 1024, 8), (64, 512, 2), (64, 512, 4), (64, 512, 6),
+                  (64, 512, 8)]
 
 
 ALIGN = 256

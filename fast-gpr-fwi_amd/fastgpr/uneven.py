@@ -38,7 +38,9 @@ from .checkpoint import History
 @dataclass
 class Problem:
     """One model's gradient request: ``G`` from ``initialization``, model
-    tensors (NI, NJ, 1), observed traces (S, nt, nrx), variant ``tb``."""
+    tensors (NI, NJ, 1), observed traces (S, nt, nrx), variant ``tb`` (a
+    multi-problem variant, or "auto": the autotuner's fastest multi-problem
+    variant for this grid and shot count, one batch on one stream)."""
     G: object
     er: torch.Tensor
     se: torch.Tensor
@@ -50,6 +52,15 @@ class Problem:
         G = self.G
         return (G.NI, G.NJ, G.S, G.iterations, int(G.receiver.shape[1]), G.pmlthick,
                 tuple(self.tb))
+
+
+def tune(p: Problem) -> tuple:
+    """The autotuner's fastest multi-problem variant for a problem's grid and
+    shots, timed as one batch on one stream (how a job runs in a wave)."""
+    import dataclasses
+    from . import autotune
+    G = dataclasses.replace(p.G, options=dict(p.G.options, shot_streams=1))
+    return tuple(autotune.choose(G, p.er, p.se, p.mr, candidates=_b.MULTI_VARIANTS))
 
 
 def plan_waves(specs, budget: int):
@@ -87,6 +98,9 @@ class UnevenBatch:
     def __init__(self, device, problems, budget: int | None = None, streams: int | None = None):
         self.device = torch.device(device)
         self.problems = list(problems)
+        for p in self.problems:
+            if p.tb == "auto":
+                p.tb = tune(p)
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             free += torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)

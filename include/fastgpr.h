@@ -229,7 +229,7 @@ int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* strea
  *   fgpr_reduce_shots    grad_eps/grad_sig += sum over shots of gshot, in shot
  *                       order (also the deferred reduction of shot groups,
  *                       see grad_eps above).
- * Supported variants: (32,512,{2,4,6}), (64,1024,{2,4,6}), (64,512,{2,4,6,8}).
+ * Supported variants: (32,512,{2,4,6}), (64,1024,{2,4,6,8}), (64,512,{2,4,6,8}).
  */
 typedef struct fgpr_tb_item {
     const fgpr_problem* prob;   /* host pointer to the problem descriptor */

@@ -1,7 +1,8 @@
 """Host logic of the time-blocked autotuner (fastgpr.autotune.choose): the
 fastest supported variant wins, variants the library does not compile are
 skipped, and winners are cached per (device, grid, P, S, receivers, shot
-groups) in the process and in the FASTGPR_TUNE_CACHE JSON file.  The
+groups, candidate set) in the process and in the FASTGPR_TUNE_CACHE JSON
+file.  The
 timing itself runs on the GPU; here it is replaced by a table."""
 import json
 from types import SimpleNamespace
@@ -14,6 +15,7 @@ from fastgpr import autotune
 
 FAKE_US = {(64, 1024, 6): 11.3, (64, 1024, 8): 12.3, (32, 256, 4): 20.0}
 CANDIDATES = [(32, 256, 4), (48, 256, 6), (64, 1024, 8), (64, 1024, 6)]   # (48, 256, 6): not compiled
+CAND = "|32,256,4;48,256,6;64,1024,6;64,1024,8"                           # the set, sorted, in the key
 
 
 @pytest.fixture
@@ -44,7 +46,7 @@ def test_fastest_supported_variant_wins_and_is_cached(tuner):
     assert autotune.choose(G, None, None, None, candidates=CANDIDATES) == (64, 1024, 6)
     # the variant the library does not compile is never timed
     assert sorted(tb for _, tb in calls) == sorted(FAKE_US)
-    key = "gfx950-test|400x200|P20|S13|rx80|g3"
+    key = "gfx950-test|400x200|P20|S13|rx80|g3" + CAND
     assert json.loads(path.read_text()) == {key: [64, 1024, 6]}
 
     n = len(calls)
@@ -64,16 +66,29 @@ def test_shot_groups_are_part_of_the_key(tuner):
     autotune.choose(grid(shot_streams=3, S=2), None, None, None, candidates=CANDIDATES)
     assert [g for g, _ in calls] == [3] * 3 + [1] * 3 + [3] * 3
     keys = sorted(json.loads(path.read_text()))
-    assert keys == ["gfx950-test|400x200|P20|S13|rx80|g1",
-                    "gfx950-test|400x200|P20|S13|rx80|g3",
-                    "gfx950-test|400x200|P20|S2|rx80|g2"]
+    assert keys == ["gfx950-test|400x200|P20|S13|rx80|g1" + CAND,
+                    "gfx950-test|400x200|P20|S13|rx80|g3" + CAND,
+                    "gfx950-test|400x200|P20|S2|rx80|g2" + CAND]
+
+
+def test_candidate_set_is_part_of_the_key(tuner):
+    """A winner among a narrower set (the multi-problem variants) is not
+    taken for the answer over all variants, nor the other way round."""
+    calls, path = tuner
+    G = grid()
+    assert autotune.choose(G, None, None, None, candidates=[(64, 1024, 8), (32, 256, 4)]) == (64, 1024, 8)
+    assert autotune.choose(G, None, None, None, candidates=CANDIDATES) == (64, 1024, 6)
+    assert len(calls) == 2 + 3
+    assert set(json.loads(path.read_text())) == {
+        "gfx950-test|400x200|P20|S13|rx80|g3|32,256,4;64,1024,8",
+        "gfx950-test|400x200|P20|S13|rx80|g3" + CAND}
 
 
 def test_hardware_queue_limit_caps_the_groups(tuner, monkeypatch):
     calls, path = tuner
     monkeypatch.setenv("GPU_MAX_HW_QUEUES", "2")     # one side stream with its own queue
     autotune.choose(grid(shot_streams=3), None, None, None, candidates=CANDIDATES)
-    assert list(json.loads(path.read_text())) == ["gfx950-test|400x200|P20|S13|rx80|g1"]
+    assert list(json.loads(path.read_text())) == ["gfx950-test|400x200|P20|S13|rx80|g1" + CAND]
 
 
 def test_timing_reps_shrink_with_the_block_and_stay_bounded():

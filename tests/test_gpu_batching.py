@@ -161,3 +161,33 @@ def test_uneven_batch_waves_reuse_memory_and_match_compute(hip_device):
             assert np.array_equal(res[idx]["traces"].cpu().numpy(), tr), (rep, idx)
             for k in ("er", "se"):
                 assert np.array_equal(res[idx][k].cpu().numpy()[..., 0], g[k]), (rep, idx, k)
+
+
+def test_uneven_batch_with_tuned_and_eight_step_variants(hip_device):
+    """Problems asking for "auto" get the autotuner's pick among the
+    multi-problem variants, one asks for the 8-step 64-row variant; every
+    result is bitwise compute()'s with the variant its job runs."""
+    from fastgpr.batching import MULTI_VARIANTS
+    from fastgpr.uneven import Problem, UnevenBatch
+    dev = hip_device
+    cases = _cases()
+    asked = ["auto", (64, 1024, 8), "auto"]
+    obs = [torch.tensor(run_product(wl, dev, time_block=(64, 1024, 6))[0]).to(dev)
+           for wl, _, _ in cases]
+    inits = [(wl.er * 0.97, wl.se * 1.1) for wl, _, _ in cases]
+    probs = []
+    for (wl, _, _), o, init, tb in zip(cases, obs, inits, asked):
+        G, er, se, mr = _job_inputs(wl, dev, init)
+        probs.append(Problem(G, er, se, mr, o, tb))
+    ub = UnevenBatch(dev, probs)
+    chosen = [p.tb for p in ub.problems]
+    print("variants", chosen, flush=True)
+    assert chosen[1] == (64, 1024, 8) and all(tuple(t) in MULTI_VARIANTS for t in chosen)
+    res = ub.run()
+    torch.cuda.synchronize()
+    for idx, ((wl, _, _), o, init) in enumerate(zip(cases, obs, inits)):
+        resid = (res[idx]["traces"] - o).cpu().numpy()
+        tr, g = run_product(wl, dev, resid=resid, model=init, time_block=chosen[idx])
+        assert np.array_equal(res[idx]["traces"].cpu().numpy(), tr), idx
+        for k in ("er", "se"):
+            assert np.array_equal(res[idx][k].cpu().numpy()[..., 0], g[k]), (idx, k)
