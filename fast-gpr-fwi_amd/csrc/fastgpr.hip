@@ -622,6 +622,7 @@ __global__ void __launch_bounds__(1024) k_normalise(const double* __restrict__ g
 constexpr int kTBCols = 64;
 constexpr int kPmax = 128;
 constexpr int kWaveList = 128;   // receiver entries one wave can hold
+constexpr unsigned kNoEntry = 0xFFFFFFFFu;   // empty slot of a receiver block
 
 // Snapshot block of one (slot, shot, tile) in the time-blocked layout of
 // variant (RI, NT, K): waves kWaveLo .. kWaveHi are those that own a
@@ -702,8 +703,9 @@ struct TBArgs {
     int ti_count, tj_count;
     int int_ti0, int_ni, int_tj0, int_nj;   // rectangle of interior tiles (no CPML cell, no edge)
     FastDiv fd_S, fd_nj, fd_per, fd_tj;     // S, int_nj, tj_count - int_nj, tj_count
-    const int* rx_ptr;   // CSR over S*ntiles: receivers sampled (forward) / injected (adjoint)
-    const int* rx_idx;   // entries s*nrx + r
+    const unsigned* rx_list;   // per (shot, tile) list_cap entries: receivers sampled (forward) /
+                               // injected (adjoint), li << 26 | lj << 20 | r, then kNoEntry
+    int list_cap;
     double* gshot;       // adjoint: S x 2 x NI x NJ per-shot partial gradients (shot-major)
 };
 
@@ -752,6 +754,18 @@ __host__ __device__ __forceinline__ bool tb_item(const TBArgs& a, int S, int wg,
     tile = (a.int_ti0 + ni) * tjc + (u - ni * per);
     return false;
 }
+
+// Device buffers are global memory.  Pointers read from a launch table (the
+// multi-problem kernel) carry no address space, and the compiler would reach
+// them with flat instructions, which also count against the LDS counter: every
+// LDS wait -- each barrier -- would then wait for the outstanding global loads
+// and stores as well.  The time-blocked kernels access their buffers through
+// this cast instead.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+typedef float fvec4 __attribute__((ext_vector_type(4)));   // 16-byte vector accesses
 
 // Load through the scalar cache (a wave-uniform address of read-only data).
 __device__ __forceinline__ int sload(const int* p) {
@@ -821,24 +835,24 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     auto xq_e = [&](int gi) -> int {
         return (gi >= 1 && gi <= P) ? P - gi : (gi >= NI - P ? gi - (NI - P) : -1);
     };
-    const float* pin = a.psi_in;
+    const auto* const pin = gmem(a.psi_in);
     const size_t xs = xslab_floats(d), ys = yslab_floats(d);
 
-    // Set-up issues every independent load before the first wait: the
-    // receiver list bounds (scalar loads) and its first 64 entries, the
-    // region's fields, coefficients and CPML memory, then the dependent
-    // receiver positions and the CPML coefficient table.
-    const int lst = a.rx_ptr ? sload(a.rx_ptr + s * ntiles + tile) : 0;
-    const int lcount = (a.rx_ptr ? sload(a.rx_ptr + s * ntiles + tile + 1) : 0) - lst;
-    const int t_first = lane < lcount ? a.rx_idx[lst + lane] : -1;
-    // CPML coefficient table entry of this thread (threads [0, 2P)), stored
-    // to LDS once every load is in
-    float4 xcv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if ((int)threadIdx.x < 2 * P) {
-        const bool h = (int)threadIdx.x < P;
-        const float* b = d.pml + (h ? 4 * P : -P) + (int)threadIdx.x;
-        xcv = make_float4(b[0] - 1.0f, b[P], b[2 * P], b[3 * P]);
-    }
+    // Set-up: every load of the region depends only on the launch arguments
+    // and the work item -- the fields, coefficients and CPML memory, this
+    // (shot, tile)'s receiver block (self-contained entries: no load waits on
+    // another) and the CPML coefficient table -- so all of them are in
+    // flight together, one memory round trip.
+    // A wave whose rows hold no CPML row, no row 0 and nothing outside the
+    // domain has clean rows; a region whose columns hold no CPML column, no
+    // column 0 and nothing outside the domain has clean columns.  Each wave
+    // runs the copy of the step loop compiled for its case, the branch-free
+    // one when both are clean; every copy meets the same barriers.  Only the
+    // CPML terms and sources use cE4 / cH4, so clean waves load them only
+    // at source cells.
+    const bool rows_clean = gi0 + r0 >= P + 1 && gi0 + r0 + CPT <= NI - P;   // wave-uniform
+    const bool cols_clean = gj0 >= P + 1 && gj0 + 64 <= NJ - P;              // region-uniform
+    const bool need4 = !(rows_clean && cols_clean);
 
     float ps[4][CPT];
 #pragma unroll
@@ -846,15 +860,16 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
         const int r = r0 + m, gi = gi0 + r;
         const bool ex = colx && gi >= 0 && gi <= NI;
         const size_t g = shot_off + (size_t)gi * pitch + gj;
-        ez[m] = ex ? a.Ez_in[g] : 0.f;
-        hx[m] = ex ? a.Hx_in[g] : 0.f;
-        hy[m] = ex ? a.Hy_in[g] : 0.f;
+        ez[m] = ex ? gmem(a.Ez_in)[g] : 0.f;
+        hx[m] = ex ? gmem(a.Hx_in)[g] : 0.f;
+        hy[m] = ex ? gmem(a.Hy_in)[g] : 0.f;
         const size_t c = (size_t)gi * pitch + gj;
-        cE0[m] = ex ? d.coef[c] : 0.f;
-        cE1[m] = ex ? d.coef[pl + c] : 0.f;
-        cE4[m] = ex ? d.coef[2 * pl + c] : 0.f;
-        cH1[m] = ex ? d.coef[3 * pl + c] : 0.f;
-        cH4[m] = ex ? d.coef[4 * pl + c] : 0.f;
+        const auto* const cf = gmem(d.coef);
+        cE0[m] = ex ? cf[c] : 0.f;
+        cE1[m] = ex ? cf[pl + c] : 0.f;
+        cE4[m] = (ex && need4) ? cf[2 * pl + c] : 0.f;
+        cH1[m] = ex ? cf[3 * pl + c] : 0.f;
+        cH4[m] = (ex && need4) ? cf[4 * pl + c] : 0.f;
         const bool ac = colact && gi >= 0 && gi < NI;
         if (ac) act |= 1u << m;
         const int qh = (ac && P > 0) ? xq_h(gi) : -1;
@@ -864,24 +879,32 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
         ps[2][m] = (ac && yqh >= 0) ? pin[4 * xs + (size_t)(yh_lo ? 2 : 3) * ys + ((size_t)s * NI + gi) * P + yqh] : 0.f;
         ps[3][m] = (ac && yqe >= 0) ? pin[4 * xs + (size_t)(ye_lo ? 0 : 1) * ys + ((size_t)s * NI + gi) * P + yqe] : 0.f;
     }
+    const int cap = a.rx_list ? a.list_cap : 0;
+    const auto* const blk = gmem(a.rx_list + (size_t)(s * ntiles + tile) * cap);
+    const unsigned e_first = cap > 0 ? blk[lane] : kNoEntry;
+    // CPML coefficient table entry of this thread (threads [0, 2P)), stored
+    // to LDS once every load is in
+    float4 xcv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((int)threadIdx.x < 2 * P) {
+        const bool h = (int)threadIdx.x < P;
+        const auto* b = gmem(d.pml + (h ? 4 * P : -P) + (int)threadIdx.x);
+        xcv = make_float4(b[0] - 1.0f, b[P], b[2 * P], b[3 * P]);
+    }
 
     // This wave's receivers (forward: tile-centre ones; adjoint: region
-    // ones), compacted in list order with a ballot per 64 entries.
+    // ones), compacted in list order with a ballot per 64 entries; the block
+    // is packed, so a chunk with an empty entry is its last.
     int wn = 0;
-    for (int base = 0; base < lcount; base += 64) {
-        const int e = base + lane;
-        const int t = base == 0 ? t_first : (e < lcount ? a.rx_idx[lst + e] : -1);
-        int ent = -1;
-        if (t >= 0) {
-            const int2 pos = *(const int2*)(d.rx + 2 * t);
-            const int li = pos.x - gi0, lj = pos.y - gj0;
-            if (li >= r0 && li < r0 + CPT && lj >= 0 && lj < 64)
-                ent = ((li - r0) << 26) | (lj << 20) | (t - (t / d.nrx) * d.nrx);
-        }
+    for (int base = 0; base < cap; base += 64) {
+        const unsigned e = base == 0 ? e_first : blk[base + lane];
+        const bool live = e != kNoEntry;
+        const int li = (int)(e >> 26);
+        const int ent = (live && li >= r0 && li < r0 + CPT) ? (((li - r0) << 26) | (int)(e & 0x3FFFFFFu)) : -1;
         const unsigned long long bal = __ballot(ent >= 0);
         const int pos = wn + (int)__popcll(bal & ((1ull << lane) - 1ull));
         if (ent >= 0 && pos < kWaveList) sRx[w][pos] = ent;
         wn += (int)__popcll(bal);
+        if (__ballot(live) != ~0ull) break;
     }
     wn = wn < kWaveList ? wn : kWaveList;
 
@@ -913,7 +936,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
 #pragma unroll
         for (int c = 0; c < kWaveList / 64; ++c) {
             if (c * 64 + lane < wn) {
-                const float* rp = d.resid + (size_t)s * d.nt * d.nrx + (rxent[c] & 0xFFFFF);
+                const auto* rp = gmem(d.resid + (size_t)s * d.nt * d.nrx + (rxent[c] & 0xFFFFF));
                 float rv[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) rv[k] = k < nsteps ? rp[(size_t)(n0 - k) * d.nrx] : 0.f;
@@ -927,12 +950,17 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     unsigned srcbits = 0u;
     if (!ADJ) {
         for (int kk = 0; kk < d.nsrc; ++kk) {
-            const int* sc = d.src + 2 * ((size_t)s * d.nsrc + kk);
+            const auto* sc = gmem(d.src + 2 * ((size_t)s * d.nsrc + kk));
             const int li = sc[0] - gi0 - r0;
             if (sc[1] == gj && li >= 0 && li < CPT) srcbits |= 1u << li;
         }
     }
     const bool src_any = __ballot(srcbits != 0u) != 0ull;   // wave-uniform
+    if (!ADJ && src_any && !need4) {
+#pragma unroll
+        for (int m = 0; m < CPT; ++m)
+            if (srcbits & (1u << m)) cE4[m] = gmem(d.coef)[2 * pl + (size_t)(gi0 + r0 + m) * pitch + gj];
+    }
 
     const float idx = d.idx;
     double ae[CPT], as_[CPT];
@@ -1039,14 +1067,6 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
         return v;
     };
 
-    // A wave whose rows hold no CPML row, no row 0 and nothing outside the
-    // domain has clean rows; a region whose columns hold no CPML column, no
-    // column 0 and nothing outside the domain has clean columns.  Each wave
-    // runs the copy of the step loop compiled for its case, the branch-free
-    // one when both are clean; every copy meets the same barriers.
-    const bool rows_clean = gi0 + r0 >= P + 1 && gi0 + r0 + CPT <= NI - P;   // wave-uniform
-    const bool cols_clean = gj0 >= P + 1 && gj0 + 64 <= NJ - P;              // region-uniform
-
     auto h_fast = [&]() {
         const float ez_below = (w + 1 < W) ? sEzTop[(w + 1) * 64 + lane] : 0.f;
 #pragma unroll
@@ -1068,8 +1088,8 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     auto slot_of = [&](int n) -> int { return d.snap_mod ? n % d.snap_mod : n - d.snap_base; };
     auto slot_next = [&](int sl) -> int { return (d.snap_mod && sl + 1 == d.snap_mod) ? 0 : sl + 1; };
     auto slot_prev = [&](int sl) -> int { return (d.snap_mod && sl == 0) ? d.snap_mod - 1 : sl - 1; };
-    auto snap_ref = [&](int sl, int m) -> float& {
-        return (snap_wave + (int64_t)sl * snap_slot + m * pitch)[gj];
+    auto snap_ref = [&](int sl, int m) -> __attribute__((address_space(1))) float& {
+        return gmem(snap_wave + (int64_t)sl * snap_slot + m * pitch)[gj];
     };
     // Time-blocked layout: per (slot, shot, tile) a block holding the waves
     // that own tile-centre rows and the tile-centre lanes only, ordered
@@ -1086,22 +1106,23 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
                                    ((w - kWlo) * TJ + (lane - K)) * CPT
                              : nullptr;
     auto tstore = [&](int sl) {
-        float* q = tsnap + sl * tslot;
+        auto* const q = gmem(tsnap + sl * tslot);
 #pragma unroll
         for (int m = 0; m < CPT; m += 4) {
             if constexpr (CPT % 4 == 0)
-                *(float4*)(q + m) = make_float4(ez[m], ez[m + 1], ez[m + 2], ez[m + 3]);
+                *(__attribute__((address_space(1))) fvec4*)(q + m) =
+                    fvec4{ez[m], ez[m + 1], ez[m + 2], ez[m + 3]};
         }
         if constexpr (CPT % 4 != 0)
 #pragma unroll
             for (int m = 0; m < CPT; ++m) q[m] = ez[m];
     };
     auto tload = [&](int sl, float* f) {
-        const float* q = tsnap + sl * tslot;
+        const auto* const q = gmem(tsnap + sl * tslot);
 #pragma unroll
         for (int m = 0; m < CPT; m += 4) {
             if constexpr (CPT % 4 == 0) {
-                const float4 v = *(const float4*)(q + m);
+                const fvec4 v = *(const __attribute__((address_space(1))) fvec4*)(q + m);
                 f[m] = v.x; f[m + 1] = v.y; f[m + 2] = v.z; f[m + 3] = v.w;
             }
         }
@@ -1152,7 +1173,7 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
 #pragma unroll
                             for (int m = 0; m < CPT; ++m) v = (m == me) ? ez[m] : v;
                             const TBArgs* ar = reload(ap);
-                            float* const tr = ld_arg(&ar->d.traces);
+                            auto* const tr = gmem(ld_arg(&ar->d.traces));
                             tr[((size_t)s * ld_arg(&ar->d.nt) + n) * ld_arg(&ar->d.nrx) + (ent & 0xFFFFF)] = v;
                         }
                     }
@@ -1175,10 +1196,10 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
                     if (src_any && (srcbits & (1u << m))) {
                         const TBArgs* ar = reload(ap);
                         const int ns = ld_arg(&ar->d.nsrc), snt = ld_arg(&ar->d.nt);
-                        const int* const src = ld_arg(&ar->d.src);
-                        const float* const ssc = ld_arg(&ar->d.srcscale);
+                        const auto* const src = gmem(ld_arg(&ar->d.src));
+                        const auto* const ssc = gmem(ld_arg(&ar->d.srcscale));
                         for (int kk = 0; kk < ns; ++kk) {
-                            const int* sc = src + 2 * ((size_t)s * ns + kk);
+                            const auto* sc = src + 2 * ((size_t)s * ns + kk);
                             if (sc[0] == gi && sc[1] == gj) v = fmaf(-cE4[m], ssc[(size_t)kk * snt + n], v);
                         }
                     }
@@ -1271,28 +1292,15 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     // output pointers and the sizes the write-back indexes with, read here:
     // nothing of the write-back is held in scalar registers through the steps
     const TBArgs* ar = reload(ap);
-    float* const pout = ld_arg(&ar->psi_out);
-    float* const Ez_out = ld_arg(&ar->Ez_out);
-    float* const Hx_out = ld_arg(&ar->Hx_out);
-    float* const Hy_out = ld_arg(&ar->Hy_out);
-    double* const gshot = ADJ ? ld_arg(&ar->gshot) : nullptr;
+    auto* const pout = gmem(ld_arg(&ar->psi_out));
+    auto* const Ez_out = gmem(ld_arg(&ar->Ez_out));
+    auto* const Hx_out = gmem(ld_arg(&ar->Hx_out));
+    auto* const Hy_out = gmem(ld_arg(&ar->Hy_out));
+    auto* const gshot = gmem(ADJ ? ld_arg(&ar->gshot) : nullptr);
     const int wNI = ld_arg(&ar->d.NI), wNJ = ld_arg(&ar->d.NJ), wP = ld_arg(&ar->d.P);
     const int wS = ld_arg(&ar->d.S), wpitch = ld_arg(&ar->d.pitch);
     const size_t wxs = (size_t)wS * wP * wNJ, wys = (size_t)wS * wNI * wP;
     const size_t wshot = (size_t)s * (size_t)(wNI + 1) * wpitch;
-    // adjoint: read every partial-gradient pair first (one round trip), add
-    // after the field stores are issued
-    double g0v[ADJ ? CPT : 1], g1v[ADJ ? CPT : 1];
-    if constexpr (ADJ) {
-#pragma unroll
-        for (int m = 0; m < CPT; ++m) {
-            const int r = r0 + m, gi = gi0 + r;
-            const bool own = r >= K && r < K + TI && centre_col && (act & (1u << m));
-            const double* gp = gshot + ((size_t)2 * s * wNI + (own ? gi : 0)) * wNJ + (own ? gj : 0);
-            g0v[m] = own ? gp[0] : 0.0;
-            g1v[m] = own ? gp[(size_t)wNI * wNJ] : 0.0;
-        }
-    }
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
         const int r = r0 + m, gi = gi0 + r;
@@ -1312,9 +1320,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
         if (yqh >= 0) pout[4 * wxs + (size_t)(yh_lo ? 2 : 3) * wys + ((size_t)s * wNI + gi) * wP + yqh] = sPsi[2][o];
         if (yqe >= 0) pout[4 * wxs + (size_t)(ye_lo ? 0 : 1) * wys + ((size_t)s * wNI + gi) * wP + yqe] = sPsi[3][o];
         if constexpr (ADJ) {
-            double* g0 = gshot + ((size_t)2 * s * wNI + gi) * wNJ + gj;
-            g0[0] = g0v[m] + ae[m];
-            g0[(size_t)wNI * wNJ] = g1v[m] + as_[m];
+            // Each cell of a shot has one owner per launch (the tile whose
+            // centre holds it) and launches are ordered, so the sum is the
+            // same as a load-add-store; the add happens at the memory side
+            // and the workgroup does not wait for a read.
+            auto* const g0 = gshot + ((size_t)2 * s * wNI + gi) * wNJ + gj;
+            __builtin_amdgcn_global_atomic_fadd_f64(g0, ae[m]);
+            __builtin_amdgcn_global_atomic_fadd_f64(g0 + (size_t)wNI * wNJ, as_[m]);
         }
     }
 }
@@ -1508,8 +1520,10 @@ int tb_check(const fgpr_problem& p, bool adj) {
     if (p.P > kPmax) return fail(FGPR_EINVAL, "CPML thicker than 128 cells");
     if (p.nrx >= (1 << 20)) return fail(FGPR_EINVAL, "time-blocked path holds at most 2^20 - 1 receivers per shot");
     if (adj && !p.gshot) return fail(FGPR_EINVAL, "time-blocked adjoint needs gshot");
-    if ((adj ? !p.tb_inj_ptr : !p.tb_rx_ptr) && p.nrx > 0)
-        return fail(FGPR_EINVAL, "time-blocked path needs the per-tile receiver tables");
+    if ((adj ? !p.tb_inj_list : !p.tb_rx_list) && p.nrx > 0)
+        return fail(FGPR_EINVAL, "time-blocked path needs the per-tile receiver blocks");
+    if (p.nrx > 0 && p.tb_list_cap != 64 && p.tb_list_cap != 128)
+        return fail(FGPR_EINVAL, "tb_list_cap must be 64 or 128");
     return FGPR_OK;
 }
 
@@ -1540,8 +1554,8 @@ TBArgs make_tb(const fgpr_problem& p, const Dev& d, int parity, bool adj) {
         range(p.P + 1 + p.tb_k, p.NJ - p.P - kTBCols + p.tb_k, TJ, p.tb_tj, a.int_tj0, a.int_nj);
         if (a.int_ni == 0 || a.int_nj == 0) { a.int_ti0 = 0; a.int_ni = 0; a.int_tj0 = 0; a.int_nj = 0; }
     }
-    a.rx_ptr = adj ? p.tb_inj_ptr : p.tb_rx_ptr;
-    a.rx_idx = adj ? p.tb_inj_idx : p.tb_rx_idx;
+    a.rx_list = (const unsigned*)(p.nrx > 0 ? (adj ? p.tb_inj_list : p.tb_rx_list) : nullptr);
+    a.list_cap = p.tb_list_cap;
     a.gshot = p.gshot;
     a.fd_S = make_fastdiv(p.S);
     a.fd_nj = make_fastdiv(a.int_nj);
@@ -1581,7 +1595,7 @@ int launch_tb(const fgpr_problem& p, const Dev& d, bool adj, int parity, int kof
 // recently launched one is destroyed once its last launch has finished (an
 // event recorded after every launch), so callers whose buffers move between
 // calls do not accumulate graphs.
-static_assert(sizeof(fgpr_problem) == 280, "a descriptor field was added: extend make_key");
+static_assert(sizeof(fgpr_problem) == 264, "a descriptor field was added: extend make_key");
 
 struct GraphKey {
     std::vector<int64_t> v;
@@ -1610,8 +1624,8 @@ GraphKey make_key(const fgpr_problem& p, int kind, int chunk) {
     P(p.coef); P(p.pml); P(p.src); P(p.srcscale); P(p.rx); P(p.Ez); P(p.Hx); P(p.Hy); P(p.psi);
     P(p.traces); P(p.snaps); P(p.resid); P(p.grad_eps); P(p.grad_sig); P(p.ctr);
     P(p.Ez2); P(p.Hx2); P(p.Hy2); P(p.psi2);
-    I(p.tb_ri); I(p.tb_nt); I(p.tb_k); I(p.tb_ti); I(p.tb_tj);
-    P(p.tb_rx_ptr); P(p.tb_rx_idx); P(p.tb_inj_ptr); P(p.tb_inj_idx); P(p.gshot); P(p.scratch);
+    I(p.tb_ri); I(p.tb_nt); I(p.tb_k); I(p.tb_ti); I(p.tb_tj); I(p.tb_list_cap);
+    P(p.tb_rx_list); P(p.tb_inj_list); P(p.gshot); P(p.scratch);
     I(p.snap_base); I(p.nsnap); I(p.snap_mod); I(p.snap_tiled);
     I(kind); I(chunk);
     return k;

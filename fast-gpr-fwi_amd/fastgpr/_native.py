@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
@@ -45,9 +45,8 @@ class Problem(ctypes.Structure):
         ("Ez2", ctypes.c_void_p), ("Hx2", ctypes.c_void_p), ("Hy2", ctypes.c_void_p),
         ("psi2", ctypes.c_void_p),
         ("tb_ri", ctypes.c_int32), ("tb_nt", ctypes.c_int32), ("tb_k", ctypes.c_int32),
-        ("tb_ti", ctypes.c_int32), ("tb_tj", ctypes.c_int32),
-        ("tb_rx_ptr", ctypes.c_void_p), ("tb_rx_idx", ctypes.c_void_p),
-        ("tb_inj_ptr", ctypes.c_void_p), ("tb_inj_idx", ctypes.c_void_p),
+        ("tb_ti", ctypes.c_int32), ("tb_tj", ctypes.c_int32), ("tb_list_cap", ctypes.c_int32),
+        ("tb_rx_list", ctypes.c_void_p), ("tb_inj_list", ctypes.c_void_p),
         ("gshot", ctypes.c_void_p),
         ("scratch", ctypes.c_void_p),
         ("snap_base", ctypes.c_int32), ("nsnap", ctypes.c_int32), ("snap_mod", ctypes.c_int32),

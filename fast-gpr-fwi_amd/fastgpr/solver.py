@@ -72,26 +72,37 @@ def cell_coeffs(er, se, mr, dt32: float, dx32: float, adjoint: bool, device,
     return coef
 
 
-TB_MAX_LIST = 128   # receivers one time-blocked workgroup can hold (kMaxList / 2)
+TB_MAX_LIST = 128   # receivers one time-blocked region can hold (entries per wave list)
+NO_ENTRY = -1       # empty slot of a receiver block (0xFFFFFFFF)
 
 
-def _csr(keys: np.ndarray, vals: np.ndarray, nkeys: int):
-    order = np.argsort(keys, kind="stable")
-    ptr = np.zeros(nkeys + 1, np.int32)
-    np.add.at(ptr, keys + 1, 1)
-    return np.cumsum(ptr).astype(np.int32), vals[order].astype(np.int32)
+def _blocks(keys: np.ndarray, ents: np.ndarray, nkeys: int, cap: int) -> np.ndarray:
+    """Entries grouped per key into blocks of ``cap`` slots, in the given
+    (receiver) order, the rest of each block NO_ENTRY."""
+    out = np.full((max(nkeys, 1), cap), NO_ENTRY, np.int64)
+    if len(keys):
+        order = np.argsort(keys, kind="stable")
+        k = keys[order]
+        start = np.searchsorted(k, k, side="left")
+        out[k, np.arange(len(k)) - start] = ents[order]
+    return out.astype(np.uint32).view(np.int32).reshape(-1)
 
 
 def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
-    """Receiver tables of the time-blocked kernels (host, numpy).
+    """Receiver blocks of the time-blocked kernels (host, numpy).
 
     rx: (S, nrx, 2) int cell indices, i < 0 = dead.  Forward: each live
     receiver is sampled by the one tile whose centre holds it.  Adjoint: each
     receiver on an updated cell (i < NI, j < NJ) is injected by every tile
-    whose region (centre + halo) holds it."""
+    whose region (centre + halo) holds it.  Per (shot, tile) a block of
+    ``cap`` (64 or 128) entries li << 26 | lj << 20 | r, the receiver's cell
+    relative to the region's first row and column and its index, in receiver
+    order (the order the adjoint applies receivers sharing a cell in)."""
     ri, nt_, k = tb
     TI, TJ, ti, tj = _native.tb_tiles(NI, NJ, ri, nt_, k)
     S, nrx = rx.shape[0], rx.shape[1]
+    if nrx >= 1 << 20:
+        raise ValueError("the time-blocked path holds at most 2^20 - 1 receivers per shot")
     ntiles = ti * tj
     fk, fv, ak, av = [], [], [], []
     for s in range(S):
@@ -99,9 +110,9 @@ def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
             i, j = int(rx[s, r, 0]), int(rx[s, r, 1])
             if i < 0:
                 continue
-            ent = s * nrx + r
-            fk.append(s * ntiles + (i // TI) * tj + (j // TJ))
-            fv.append(ent)
+            a, b = i // TI, j // TJ
+            fk.append(s * ntiles + a * tj + b)
+            fv.append(((i - a * TI + k) << 26) | ((j - b * TJ + k) << 20) | r)
             if i >= NI or j >= NJ:
                 continue
             for a in range(max(0, (i + k - ri) // TI), min(ti - 1, (i + k) // TI) + 1):
@@ -110,14 +121,17 @@ def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
                 for b in range(max(0, (j + k - 64) // TJ), min(tj - 1, (j + k) // TJ) + 1):
                     if b * TJ - k <= j < b * TJ - k + 64:
                         ak.append(s * ntiles + a * tj + b)
-                        av.append(ent)
-    fptr, fidx = _csr(np.asarray(fk, np.int64), np.asarray(fv, np.int64), S * ntiles)
-    aptr, aidx = _csr(np.asarray(ak, np.int64), np.asarray(av, np.int64), S * ntiles)
-    most = max(int(np.diff(fptr).max(initial=0)), int(np.diff(aptr).max(initial=0)))
+                        av.append(((i - a * TI + k) << 26) | ((j - b * TJ + k) << 20) | r)
+    fk, ak = np.asarray(fk, np.int64), np.asarray(ak, np.int64)
+    most = max(int(np.bincount(fk, minlength=1).max()) if len(fk) else 0,
+               int(np.bincount(ak, minlength=1).max()) if len(ak) else 0)
     if most > TB_MAX_LIST:
         raise ValueError(f"time-blocked variant {tuple(tb)}: {most} receivers in one tile region "
                          f"(at most {TB_MAX_LIST}); use smaller regions or the per-step kernels")
-    return dict(TI=TI, TJ=TJ, ti=ti, tj=tj, fptr=fptr, fidx=fidx, aptr=aptr, aidx=aidx)
+    cap = 64 if most <= 64 else 128
+    return dict(TI=TI, TJ=TJ, ti=ti, tj=tj, cap=cap,
+                fwd=_blocks(fk, np.asarray(fv, np.int64), S * ntiles, cap),
+                adj=_blocks(ak, np.asarray(av, np.int64), S * ntiles, cap))
 
 
 class Batch:
@@ -173,8 +187,8 @@ class Batch:
             t = tile_tables(rx_host.numpy(), self.NI, self.NJ, self.tb)
             self.tiles = (t["ti"], t["tj"])
             as_dev = lambda a: to_device(torch.from_numpy(np.ascontiguousarray(a)), dev)
-            self.tb_tables = [as_dev(t["fptr"]), as_dev(t["fidx"] if len(t["fidx"]) else np.zeros(1, np.int32)),
-                              as_dev(t["aptr"]), as_dev(t["aidx"] if len(t["aidx"]) else np.zeros(1, np.int32))]
+            self.tb_tables = [as_dev(t["fwd"]), as_dev(t["adj"])]
+            self.tb_list_cap = t["cap"]
         empty = ((lambda shape, dt: device_empty(shape, dt, dev)) if alloc is None
                  else alloc)
         self.snaps = empty((self.nt,) + self.snap_slot_shape(), torch.float32) if store else None
@@ -244,7 +258,8 @@ class Batch:
             p.psi2 = self.psi2.data_ptr()
             p.tb_ri, p.tb_nt, p.tb_k = self.tb
             p.tb_ti, p.tb_tj = self.tiles
-            p.tb_rx_ptr, p.tb_rx_idx, p.tb_inj_ptr, p.tb_inj_idx = [t.data_ptr() for t in self.tb_tables]
+            p.tb_rx_list, p.tb_inj_list = [t.data_ptr() for t in self.tb_tables]
+            p.tb_list_cap = self.tb_list_cap
             p.gshot = self.gshot.data_ptr() if self.gshot is not None else None
         return p
 

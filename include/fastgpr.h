@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 10
+#define FGPR_ABI_VERSION 11
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -113,10 +113,14 @@ typedef struct fgpr_problem {
     float* psi2;
     int32_t tb_ri, tb_nt, tb_k; /* region rows, threads per workgroup, steps per launch */
     int32_t tb_ti, tb_tj;       /* tiles along i and j */
-    const int32_t* tb_rx_ptr;   /* forward: receivers each (shot, tile) samples, CSR over S*tiles */
-    const int32_t* tb_rx_idx;   /*   entries s*nrx + r */
-    const int32_t* tb_inj_ptr;  /* adjoint: receivers inside each (shot, tile) region */
-    const int32_t* tb_inj_idx;
+    int32_t tb_list_cap;        /* entries per (shot, tile) receiver block: 64 or 128 */
+    const int32_t* tb_rx_list;  /* forward: per (shot, tile) a block of tb_list_cap entries, the
+                                   receivers the tile centre samples in receiver order, then
+                                   0xFFFFFFFF; entry = li << 26 | lj << 20 | r with (li, lj) the
+                                   receiver's cell relative to the region's first row and column
+                                   (tile * (TI, TJ) - tb_k) and r its index in the shot */
+    const int32_t* tb_inj_list; /* adjoint: the same blocks for the receivers inside each region
+                                   (centre and halo) */
     double* gshot;              /* adjoint: S x 2 x NI x NJ per-shot FP64 partial gradients, shot-major
                                    (a shot range is a contiguous slice); zeroed by the caller */
     float* scratch;             /* exact adjoint: 7*S*(NI+1)*pitch floats, zeroed by the caller */

@@ -126,3 +126,63 @@ def test_planted_division_error_is_caught():
         assert all(fastdiv(int(v), d) == v // d for v in n[::997])
     bad = sum(fastdiv(int(v), 51, bump=-1) != v // 51 for v in n[:: 7])
     assert bad > 0
+
+
+def _decode(e):
+    u = int(np.uint32(np.int32(e)))
+    return u >> 26, (u >> 20) & 63, u & 0xFFFFF
+
+
+@pytest.mark.parametrize("tb", [(64, 1024, 6), (64, 1024, 8), (32, 512, 4), (64, 512, 3)])
+def test_receiver_blocks_match_a_brute_force_region_scan(tb):
+    """solver.tile_tables against a scan of every region: the forward block of
+    a (shot, tile) holds exactly the live receivers in the tile centre, the
+    adjoint block exactly those on updated cells anywhere in the region;
+    entries give the cell relative to the region and the receiver index, in
+    receiver order (the order receivers on one cell are applied in), and the
+    block is packed (empty slots only at the end)."""
+    from fastgpr.solver import tile_tables
+    rng = np.random.default_rng(sum(tb))
+    NI, NJ, S, nrx = 157, 131, 3, 40
+    rx = np.stack([rng.integers(0, NI + 1, (S, nrx)), rng.integers(0, NJ + 1, (S, nrx))], -1)
+    rx[0, 3] = rx[0, 1]                                  # two receivers on one cell
+    rx[1, 5] = rx[1, 6] = rx[1, 2]                       # three
+    rx[2, 7, 0] = -1                                     # dead
+    rx[2, :4] = [(NI, 5), (7, NJ), (0, 0), (NI - 1, NJ - 1)]   # edges
+    t = tile_tables(rx, NI, NJ, tb)
+    RI, _, K = tb
+    TI, TJ, ti, tj, cap = t["TI"], t["TJ"], t["ti"], t["tj"], t["cap"]
+    assert cap == 64
+    fwd = t["fwd"].reshape(S, ti * tj, cap)
+    adj = t["adj"].reshape(S, ti * tj, cap)
+    for s in range(S):
+        for a in range(ti):
+            for b in range(tj):
+                gi0, gj0 = a * TI - K, b * TJ - K
+                want_f, want_a = [], []
+                for r in range(nrx):
+                    i, j = int(rx[s, r, 0]), int(rx[s, r, 1])
+                    if i < 0:
+                        continue
+                    if a * TI <= i < (a + 1) * TI and b * TJ <= j < (b + 1) * TJ:
+                        want_f.append((i - gi0, j - gj0, r))
+                    if i < NI and j < NJ and gi0 <= i < gi0 + RI and gj0 <= j < gj0 + 64:
+                        want_a.append((i - gi0, j - gj0, r))
+                for blk, want in ((fwd[s, a * tj + b], want_f), (adj[s, a * tj + b], want_a)):
+                    n = len(want)
+                    assert [_decode(e) for e in blk[:n]] == want, (s, a, b)
+                    assert np.all(blk[n:] == -1), (s, a, b)
+
+
+def test_receiver_blocks_grow_to_128_and_refuse_more():
+    from fastgpr.solver import tile_tables
+    rx = np.zeros((1, 100, 2), np.int64)
+    rx[0, :, 0] = 30
+    rx[0, :, 1] = np.arange(100) % 40 + 5                # 100 receivers in one region
+    t = tile_tables(rx, 200, 200, (64, 1024, 6))
+    assert t["cap"] == 128 and t["adj"].size == 128 * t["ti"] * t["tj"]
+    rx = np.zeros((1, 129, 2), np.int64)
+    rx[0, :, 0] = 30
+    rx[0, :, 1] = 20                                     # 129 on one cell
+    with pytest.raises(ValueError, match="at most 128"):
+        tile_tables(rx, 200, 200, (64, 1024, 6))
