@@ -82,6 +82,11 @@ def parse(argv=None):
     ap.add_argument("--kernel-stats", metavar="DB",
                     help="per (kernel, grid size) dispatch counts and durations of a rocprofv3 "
                          "kernel-trace database; uses no GPU")
+    ap.add_argument("--section", choices=["c3", "one", "c2", "plain", "serve"], default=None,
+                    help="run one section of the default line and print its JSON (the default "
+                         "run starts one process per section)")
+    ap.add_argument("--ref-value", type=float, default=None,
+                    help="--section plain: the headline value the plain path is compared with")
     ap.add_argument("--kernel-busy", nargs=3, metavar=("DB", "T0_NS", "T1_NS"),
                     help="read a rocprofv3 kernel-trace database and print how much of the "
                          "monotonic window [T0, T1] (the serving line's window_ns) had a kernel "
@@ -195,8 +200,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.section is None and args.workload == "c3" and not args.quick and not args.pmc_probe \
+            and args.serve <= 0:
+        sys.exit(run_sections(args))
+
     import torch
-    from fastgpr.workloads import c2_crosshole
 
     dev = torch.device("cuda:0")
     if args.pmc_probe:
@@ -205,28 +213,79 @@ def main():
     if args.serve > 0:
         print(json.dumps(serve_bench(dev, args.serve, args.rate)), flush=True)
         return
-    if args.workload == "c2":
+    if args.workload == "c2" and args.section is None:
         out = c2_line(dev, args)
         if args.plain:
             out["plain_path"] = plain_path(dev, out["value"])
         print(json.dumps(out), flush=True)
         return
-
-    out = c3_line(dev, args)
-    if not args.quick:
-        log("C3 one model at a time")
-        out["one_model_at_a_time"] = one_at_a_time(dev)
-        out["one_model_at_a_time"]["batched_speedup"] = round(
-            out["value"] / out["one_model_at_a_time"]["value"], 3)
-        log("C2 cross-hole line")
+    sec = args.section or "c3"
+    if sec == "c3":
+        out = c3_line(dev, args)
+    elif sec == "one":
+        out = one_at_a_time(dev)
+    elif sec == "c2":
         c2 = c2_line(dev, args, extras=False)
-        out["c2_crosshole"] = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline")}
-        log("plain path")
-        out["plain_path"] = plain_path(dev, out["value"])
-        if args.serve_check > 0:
-            log("serving")
-            out["serving"] = serve_bench(dev, args.serve_check, args.serve_rate)
+        out = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline")}
+    elif sec == "plain":
+        out = plain_path(dev, args.ref_value)
+    else:
+        out = serve_bench(dev, args.serve_check, args.serve_rate)
     print(json.dumps(out), flush=True)
+
+
+# Sections of the default line, each run in a process of its own (in this
+# order): the headline, then the extra keys.  A section that fails leaves an
+# "error" in its key and the others still run.
+SECTIONS = (("c3", None), ("one", "one_model_at_a_time"), ("c2", "c2_crosshole"),
+            ("plain", "plain_path"), ("serve", "serving"))
+SECTION_TIMEOUT_S = {"c3": 420, "one": 300, "c2": 240, "plain": 240, "serve": 300}
+
+
+def run_sections(args) -> int:
+    """The default line: every section in a child process of this script
+    (this process does no GPU work, so each child has the whole device),
+    merged into one JSON line.  Returns the exit status: that of the
+    headline section."""
+    import subprocess
+    here = str(Path(__file__).resolve())
+    out, status = None, 0
+    for sec, key in SECTIONS:
+        if sec == "serve" and args.serve_check <= 0:
+            continue
+        argv = [sys.executable, here, *sys.argv[1:], "--section", sec]
+        if sec == "plain":
+            if out is None or out.get("value") is None:
+                continue
+            argv += ["--ref-value", str(out["value"])]
+        log(f"section {sec}")
+        try:
+            p = subprocess.run(argv, stdout=subprocess.PIPE, text=True,
+                               timeout=SECTION_TIMEOUT_S[sec])
+            lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+            res = json.loads(lines[-1]) if lines else None
+            err = None if res is not None else f"section {sec} exited with status {p.returncode}"
+            if res is not None and p.returncode != 0:
+                # printed its result, then failed on the way out: keep the
+                # figures and say so
+                log(f"section {sec} printed its result and exited with status {p.returncode}")
+                res["exit_status"] = p.returncode
+        except subprocess.TimeoutExpired:
+            res, err = None, f"section {sec} exceeded {SECTION_TIMEOUT_S[sec]} s"
+        if err:
+            log(err)
+        if sec == "c3":
+            if res is None:
+                status = 1
+                res = {"metric": METRIC, "value": None, "unit": "Gcell-updates/s", "n_gpus": 1,
+                       "steps": args.steps, "warmup": args.warmup, "error": err}
+            out = res
+            continue
+        out[key] = res if res is not None else {"error": err}
+        if sec == "one" and res is not None and out.get("value"):
+            res["batched_speedup"] = round(out["value"] / res["value"], 3)
+    print(json.dumps(out), flush=True)
+    return status
 
 
 def log(msg):

@@ -3,8 +3,8 @@
 For one grid (NI, NJ, P) and batch size S it times every compiled variant
 (region rows x threads x steps per launch): repeated forward and adjoint
 launches of one block on scratch buffers, split into the shot groups the
-sweep will use, replayed from a captured HIP graph as the sweeps replay
-theirs, and keeps the fastest per step.  Results are cached per process and,
+sweep will use and launched back to back, and keeps the fastest per step.
+Results are cached per process and,
 when FASTGPR_TUNE_CACHE names a file, in that JSON file (written atomically).
 """
 from __future__ import annotations
@@ -106,23 +106,10 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
                 run(groups[0], ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), r)
         launches(2)
         torch.cuda.synchronize(dev)
-        # the sweeps replay captured graphs, so the variants are timed that
-        # way too (short launches run closer together than eager ones)
-        graph = torch.cuda.CUDAGraph()
-        try:
-            with torch.cuda.graph(graph):
-                launches(reps)
-        except RuntimeError:
-            graph = None
-        if graph is not None:
-            graph.replay()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(main)
-        if graph is not None:
-            graph.replay()
-        else:
-            launches(reps)
+        launches(reps)
         e1.record(main)
         e1.synchronize()
         total += e0.elapsed_time(e1) / 1e3 / reps

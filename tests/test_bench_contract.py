@@ -100,3 +100,42 @@ def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
     assert m["bytes_per_launch"] == 3 * 1000 * 1024
     assert bench.measured_traffic(tmp_path / "t.json",
                                   "c3_model3_1200x600_15shots|k_tb_multi<64,1024,6,true>") == 3072000
+
+
+def test_default_line_merges_sections_and_survives_a_failed_one(monkeypatch, capsys):
+    """The default run starts one child per section and merges their JSON
+    lines: a section that fails leaves an error in its key, one that printed
+    its result before failing keeps it with its exit status, and the plain
+    path is handed the headline value."""
+    import json
+    import subprocess as sp
+    import bench
+    seen = []
+
+    class R:
+        def __init__(self, rc, out):
+            self.returncode, self.stdout = rc, out
+
+    def fake_run(argv, stdout=None, text=None, timeout=None):
+        sec = argv[argv.index("--section") + 1]
+        seen.append((sec, argv))
+        if sec == "c3":
+            return R(0, "noise\n" + json.dumps({"metric": bench.METRIC, "value": 200.0}) + "\n")
+        if sec == "one":
+            return R(134, json.dumps({"value": 100.0}) + "\n")
+        if sec == "c2":
+            return R(1, "")
+        return R(0, json.dumps({"value": 1.0}) + "\n")
+
+    monkeypatch.setattr(sp, "run", fake_run)
+    monkeypatch.setattr(bench.sys, "argv", ["bench.py"])
+    status = bench.run_sections(bench.parse([]))
+    out = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    assert status == 0 and out["value"] == 200.0
+    assert out["one_model_at_a_time"] == {"value": 100.0, "exit_status": 134,
+                                          "batched_speedup": 2.0}
+    assert "error" in out["c2_crosshole"]
+    assert out["plain_path"] == {"value": 1.0} and out["serving"] == {"value": 1.0}
+    plain = dict(seen)["plain"]
+    assert plain[plain.index("--ref-value") + 1] == "200.0"
+    assert [s for s, _ in seen] == ["c3", "one", "c2", "plain", "serve"]
