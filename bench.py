@@ -49,6 +49,9 @@ def parse(argv=None):
     ap.add_argument("--c3-variant", default="auto",
                     help="time-blocking variant of the uneven batch's jobs: 'auto' (the autotuner's "
                          "pick per model) or rows,threads,steps for all")
+    ap.add_argument("--c3-shot-groups", type=int, default=3,
+                    help="shot-group jobs per model of the uneven batch whose history fits "
+                         "(1 = one job per model)")
     ap.add_argument("--exec", dest="exec_mode", default="graph", choices=["graph", "eager"])
     ap.add_argument("--chunk", type=int, default=None,
                     help="C2: steps per captured graph (default: the library's choice)")
@@ -340,7 +343,7 @@ def c3_line(dev, args):
     log("C3: observed traces and problems")
     wls, probs = c3_problems(dev, [tb] * 4)
     t0 = time.perf_counter()
-    ub = UnevenBatch(dev, probs)
+    ub = UnevenBatch(dev, probs, shot_groups=args.c3_shot_groups)
     build_s = time.perf_counter() - t0
     tbs = [p.tb for p in ub.problems]
     log(f"C3: built and captured in {build_s:.1f} s: {ub.summary()}")
@@ -381,6 +384,7 @@ def c3_line(dev, args):
                    "dx_m": wls[0].dx, "cpml": wls[0].P,
                    "cell_updates_per_step": cu, "time_block": [list(t) for t in tbs],
                    "exec": "one captured HIP graph per step (all ticks, all waves)",
+                   "shot_groups": args.c3_shot_groups,
                    "waves": summ["waves"], "snapshot_rings": summ["rings"],
                    "recompute_steps": summ["recompute_steps"],
                    "ticks": summ["ticks"], "launches": summ["launches"],
@@ -393,39 +397,46 @@ def c3_line(dev, args):
 
 def multi_roofline(ub, reps):
     """Average launch duration of the multi-problem kernels on the largest
-    model's job (one item per launch, as in its wave), timed with HIP events
-    on the launch stream, against the algorithmic bytes of a block and, when
+    model's block (one launch holding the items of all its shot-group jobs:
+    every shot once, the workgroups of one tick of its wave), timed with HIP
+    events on the launch stream, against the algorithmic bytes of a block and, when
     the counter runs measured this exact kernel and workload, the HBM bytes
     it moved."""
     import ctypes
     import torch
     from fastgpr import _native
     from fastgpr._native import check, lib
-    k = max(range(len(ub.jobs)), key=lambda i: ub.jobs[i].cell_updates())
-    job = ub.jobs[k]
-    tb = job.tb
-    G = job.G
+    # the largest model; its shot-group jobs together are one block of all its shots
+    work = {}
+    for j, (kk, _) in zip(ub.jobs, ub.parts):
+        work[kk] = work.get(kk, 0) + j.cell_updates()
+    k = max(work, key=work.get)
+    jobs = [j for j, (kk, _) in zip(ub.jobs, ub.parts) if kk == k]
+    tb = jobs[0].tb
+    G = ub.problems[k].G
     name = f"c3_model{k}_{G.NI}x{G.NJ}_{G.S}shots"
     st = torch.cuda.current_stream()
     sp = ctypes.c_void_p(st.cuda_stream)
     per = {}
     K = tb[2]
-    # the forward as the sweep runs it, with its snapshot stores
-    fwd = job.pf if job.gradient else job.pf_plain
-    for adj, prob, n in ((False, fwd, G.iterations - K), (True, job.pa, G.iterations - 1)):
-        item = _native.TBItem()
-        item.prob = ctypes.pointer(prob)
-        item.n0, item.nsteps, item.parity = n, K, 0
-        nbytes = lib.fgpr_tb_table_bytes(1)
+    n = len(jobs)
+    for adj, step0 in ((False, G.iterations - K), (True, G.iterations - 1)):
+        # the forward as the sweep runs it, with its snapshot stores
+        probs = [(j.pa if adj else (j.pf if j.gradient else j.pf_plain)) for j in jobs]
+        items = (_native.TBItem * n)()
+        for it, prob in zip(items, probs):
+            it.prob = ctypes.pointer(prob)
+            it.n0, it.nsteps, it.parity = step0, K, 0
+        nbytes = lib.fgpr_tb_table_bytes(n)
         host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
         total = ctypes.c_int()
-        check(lib.fgpr_tb_pack((_native.TBItem * 1)(item), 1, 1 if adj else 0,
+        check(lib.fgpr_tb_pack(items, n, 1 if adj else 0,
                                ctypes.c_void_p(host.data_ptr()), nbytes, ctypes.byref(total)))
         table = host.to(G.device)
 
         def launch(r):
             for _ in range(r):
-                check(lib.fgpr_tb_launch_table(ctypes.c_void_p(table.data_ptr()), 1, total.value,
+                check(lib.fgpr_tb_launch_table(ctypes.c_void_p(table.data_ptr()), n, total.value,
                                                tb[0], tb[1], tb[2], 1 if adj else 0, sp))
         launch(4)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -445,7 +456,8 @@ def multi_roofline(ub, reps):
             "basis": "achieved = algorithmic bytes of the block (40 B per cell-update + 16 B per "
                      "CPML cell and direction, x steps) / launch time; time blocking keeps the "
                      "fields on chip, so this is an equivalent rate -- physical is what HBM moved",
-            "kernel": f"{best} on {name} (the largest model, one problem per launch)",
+            "kernel": f"{best} on {name} (the largest model: one launch over all its shots, "
+                      f"{n} shot-group item(s))",
             "workload_key": f"{name}|{best}",
             "avg_launch_us": round(d["us"], 2), "bytes_per_launch": int(d["bytes"]),
             "launches_per_step": int(2 * G.iterations / K)}

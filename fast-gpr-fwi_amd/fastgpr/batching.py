@@ -212,10 +212,18 @@ class Job:
     large buffers (default: torch tensors); nothing is written to them until
     the job's first operation runs on its stream.  ``keep`` (default
     ``alloc``) supplies the results: the traces and the normalised
-    gradients."""
+    gradients.
+
+    ``shots`` = (lo, hi) and ``group`` (a ``ShotGroup``) make the job one
+    shot group of a model split over several jobs: it computes shots lo..hi-1
+    into its slices of the group's traces and per-shot partial gradients and
+    leaves the shot reduction and the normalisation to the group's
+    ``finish_ops`` (one pass in shot order once every group is done, as for
+    one batch).  Such a job must keep its whole snapshot history (one adjoint
+    phase)."""
 
     def __init__(self, G, er, se, mr, obs=None, tb=(64, 1024, 6), history=None,
-                 want=("er", "se"), tag=None, alloc=None, keep=None):
+                 want=("er", "se"), tag=None, alloc=None, keep=None, shots=None, group=None):
         if tuple(tb) not in MULTI_VARIANTS:
             raise ValueError(f"time-blocking variant {tb} has no multi-problem kernel")
         dev = G.device
@@ -243,8 +251,10 @@ class Job:
         self.er, self.se, self.mr = f32(er), f32(se), f32(mr)
         self.se_f = torch.empty_like(self.se)
         nrx = int(G.receiver.shape[1])
-        self.fb = _s.Batch(G, store=False, tb=self.tb, alloc=alloc,
-                           traces=keep((G.S, nt, nrx), torch.float32))
+        lo, hi = shots if shots is not None else (0, G.S)
+        self.group = group
+        traces = group.traces[lo:hi] if group is not None else keep((hi - lo, nt, nrx), torch.float32)
+        self.fb = _s.Batch(G, store=False, tb=self.tb, alloc=alloc, traces=traces, shots=(lo, hi))
         self.scale = _s.emitting_sources(G, self.fb)
         S = self.fb.S
         pitch = self.fb.pitch
@@ -264,7 +274,9 @@ class Job:
             self.phases = [("fwd", 0, nt)]
             return
         R = hist.R
-        self.obs = obs.detach().to(device=dev, dtype=torch.float32).reshape(S, nt, nrx).contiguous()
+        if group is not None and not hist.full:
+            raise ValueError("a shot group of a split model must keep its whole history")
+        self.obs = obs.detach().to(device=dev, dtype=torch.float32)[lo:hi].reshape(S, nt, nrx).contiguous()
         self.ring = alloc((R,) + self.fb.snap_slot_shape(), torch.float32)
         self.pf = self.fb.problem(self.coef_f, srcscale=self.scale)
         self.pf.snaps, self.pf.nsnap, self.pf.snap_mod = self.ring.data_ptr(), R, R
@@ -276,13 +288,20 @@ class Job:
         self.first_stored = hist.first_stored()
         self.marks = set(hist.saves()) | {self.first_stored}
         self.resid = alloc((S, nt, nrx), torch.float32)
-        self.ab = _s.Batch(G, store=False, tb=self.tb, alloc=alloc, traces=self.resid)
+        self.ab = _s.Batch(G, store=False, tb=self.tb, alloc=alloc, traces=self.resid, shots=(lo, hi))
         self.coef_a = alloc((5, NI + 1, pitch), torch.float32)
-        self.ge = alloc((NI, NJ), torch.float64)
-        self.gs = alloc((NI, NJ), torch.float64)
-        self.ab.gshot_buffer(zero=False)
-        self.out = {name: keep((NI, NJ, 1), torch.float32) for name in ("er", "se") if name in want}
-        self.pa = self.ab.problem(self.coef_a, resid=self.resid, grad_eps=self.ge, grad_sig=self.gs)
+        if group is None:
+            self.ge = alloc((NI, NJ), torch.float64)
+            self.gs = alloc((NI, NJ), torch.float64)
+            self.ab.gshot_buffer(zero=False)
+            self.out = {name: keep((NI, NJ, 1), torch.float32) for name in ("er", "se") if name in want}
+            self.pa = self.ab.problem(self.coef_a, resid=self.resid, grad_eps=self.ge, grad_sig=self.gs)
+        else:
+            # per-shot partials into the group's buffer; no gradient sums here
+            self.ge = self.gs = None
+            self.ab.gshot = group.gshot[lo:hi]
+            self.out = group.out
+            self.pa = self.ab.problem(self.coef_a, resid=self.resid)
         self.pa.snaps, self.pa.nsnap, self.pa.snap_mod = self.ring.data_ptr(), R, R
         self.phases = [("fwd", 0, nt)]
         t = hist.tail()
@@ -307,8 +326,9 @@ class Job:
                            out=self.coef_f)
             if self.gradient:
                 self.ab.reset_state()
-                self.ge.zero_()
-                self.gs.zero_()
+                if self.group is None:
+                    self.ge.zero_()
+                    self.gs.zero_()
                 _s.cell_coeffs(self.er, self.se, self.mr, G.dt32, G.dx32, True, dev,
                                out=self.coef_a)          # Ucgeta
         return [start]
@@ -401,7 +421,9 @@ class Job:
             par = self.parity
             ops.append(lambda st, par=par:
                        check(lib.fgpr_tb_settle(ctypes.byref(self.pa), par, st), "settle"))
-            ops.append(lambda st: check(lib.fgpr_reduce_shots(ctypes.byref(self.pa), st), "reduce"))
+            if self.group is None:         # a shot group's reduction is its group's
+                ops.append(lambda st: check(lib.fgpr_reduce_shots(ctypes.byref(self.pa), st),
+                                            "reduce"))
         self.ip += 1
         if self.done:
             ops.extend(self._finish_ops())
@@ -411,7 +433,7 @@ class Job:
 
     def _finish_ops(self) -> list:
         def finish(st):
-            if self.gradient:
+            if self.gradient and self.group is None:
                 NI, NJ = self.fb.NI, self.fb.NJ
                 for name, g in (("er", self.ge), ("se", self.gs)):
                     if name in self.out:
@@ -444,6 +466,63 @@ class Job:
 
     def cell_updates(self) -> int:
         return (2 if self.gradient else 1) * self.fb.S * self.fb.NI * self.fb.NJ * self.nt
+
+
+class ShotGroup:
+    """Shared state of one model split into shot-group jobs: whole-model
+    traces (S, nt, nrx), per-shot FP64 partial gradients (S, 2, NI, NJ), the
+    FP64 sums and the normalised outputs, from ``keep``.  ``finish_ops``
+    (after every group job is done) reduces the partials over all shots in
+    shot order and normalises -- what one job of all the shots does at its
+    end, so the results are bitwise the same."""
+
+    def __init__(self, G, keep, want=("er", "se")):
+        S, nt, nrx = G.S, G.iterations, int(G.receiver.shape[1])
+        self.G, self.S = G, S
+        self.traces = keep((S, nt, nrx), torch.float32)
+        self.gshot = keep((S, 2, G.NI, G.NJ), torch.float64)
+        self.ge = keep((G.NI, G.NJ), torch.float64)
+        self.gs = keep((G.NI, G.NJ), torch.float64)
+        self.out = {name: keep((G.NI, G.NJ, 1), torch.float32) for name in ("er", "se")
+                    if name in want}
+        self.template = None     # a copy of one group job's problem struct
+
+    def adopt(self, job) -> None:
+        """Take the problem struct of one of the group's jobs (its grid, CPML
+        and coefficient pointers, which every group job shares) as the
+        template of the reduction.  A copy, so the group holds no reference
+        to its jobs."""
+        if self.template is None:
+            self.template = bytes(job.pa)
+
+    @staticmethod
+    def bytes(G, want=("er", "se")) -> int:
+        """Bytes ``keep`` supplies for a group (with alignment padding)."""
+        S, nt, nrx = G.S, G.iterations, int(G.receiver.shape[1])
+        sizes = [S * nt * nrx * 4, S * 2 * G.NI * G.NJ * 8, G.NI * G.NJ * 8, G.NI * G.NJ * 8]
+        sizes += [G.NI * G.NJ * 4] * len(want)
+        return sum(_aligned(b) for b in sizes)
+
+    def finish_ops(self) -> list:
+        def finish(st):
+            self.ge.zero_()
+            self.gs.zero_()
+            p = _native.Problem.from_buffer_copy(self.template)
+            p.S = self.S
+            p.gshot = self.gshot.data_ptr()
+            p.grad_eps, p.grad_sig = self.ge.data_ptr(), self.gs.data_ptr()
+            check(lib.fgpr_reduce_shots(ctypes.byref(p), st), "reduce")
+            NI, NJ = self.G.NI, self.G.NJ
+            for name, g in (("er", self.ge), ("se", self.gs)):
+                if name in self.out:
+                    check(lib.fgpr_normalise(_vp(g), NI * NJ, 1.0, _vp(self.out[name]), st),
+                          "normalise")
+        return [finish]
+
+    def results(self) -> dict:
+        out = dict(self.out)
+        out["traces"] = self.traces
+        return out
 
 
 def _run_ops(ops, stream: torch.cuda.Stream):
@@ -580,14 +659,15 @@ class Scheduler:
             self.tick()
         self.join()
 
-    def capture(self, waves=None) -> "CapturedBatch":
+    def capture(self, waves=None, final_ops=None) -> "CapturedBatch":
         """Plan jobs to their end and capture the whole schedule -- start
         operations, every launch and every phase change, on the jobs'
         streams -- in one HIP graph.  ``waves``: lists of added (not yet
         started) jobs run one after another (all streams join between waves,
         so a wave may reuse the memory of the one before); default: all jobs
-        in one wave.  The launch tables of all ticks are packed on the host
-        and uploaded once."""
+        in one wave.  ``final_ops`` run on the capturing stream after every
+        wave has joined.  The launch tables of all ticks are packed on the
+        host and uploaded once."""
         waves = [list(self.jobs)] if waves is None else [list(w) for w in waves]
         everyone = self.jobs
         plans = []
@@ -636,6 +716,8 @@ class Scheduler:
                         k += 1
                 for st in self.streams:
                     cap.wait_stream(st)
+            if final_ops:
+                _run_ops(list(final_ops), cap)
         for wave in waves:
             for j in wave:
                 j.ip = len(j.phases)          # the host phase machines ran to the end

@@ -129,6 +129,18 @@ def state_bytes(G) -> int:
     return 4 * (3 * G.S * (G.NI + 1) * pitch + max(_native.psi_floats(G.NI, G.NJ, G.pmlthick, G.S), 1))
 
 
+def device_budget(device) -> int:
+    """Device bytes a gradient's snapshot history and working buffers may
+    take: the free memory, counting what torch's caching allocator holds
+    unused, less a reserve for everything else (5 % of the device, at least
+    8 GiB)."""
+    device = torch.device(device)
+    total = torch.cuda.get_device_properties(device).total_memory
+    free, _ = torch.cuda.mem_get_info(device)
+    free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    return int(free - max(8 << 30, 0.05 * total))
+
+
 def history(G, checkpoint) -> History | None:
     """How the snapshots of G's gradient are kept; None = every snapshot in
     one buffer (the reference's full history).
@@ -152,9 +164,7 @@ def history(G, checkpoint) -> History | None:
         total = torch.cuda.get_device_properties(G.device).total_memory
         if full <= 0.25 * total:
             return None
-        free, _ = torch.cuda.mem_get_info(G.device)
-        free += torch.cuda.memory_reserved(G.device) - torch.cuda.memory_allocated(G.device)
-        budget = int(free - max(8 << 30, 0.05 * total))
+        budget = device_budget(G.device)
         if full <= budget:
             return None
         return plan(nt, sb, state_bytes(G), budget, lowest)
