@@ -1245,15 +1245,11 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
                         lam = e_cell(xs_tag, ys_tag, m, r * 64 + lane, hyim, hxjm);
                     }
                     ez[m] = lam;
-                    // Rows outside the tile centre are skipped (a wave-uniform
-                    // test); columns are not tested: lanes outside the centre
-                    // hold zero snapshots and their sums are never written back.
-                    if (r >= K && r < K + TI) {
-                        const float fn = fcur[m];
-                        const float fp = fprv[m];
-                        ae[m] += (double)((fn - fp) * lam);
-                        as_[m] += (double)(fn * lam);
-                    }
+                    // Every row and lane accumulates (no select around the
+                    // FP64 adds): only tile-centre cells are written back, and
+                    // the other sums are never read.
+                    ae[m] += (double)((fcur[m] - fprv[m]) * lam);
+                    as_[m] += (double)(fcur[m] * lam);
                 }
                 // snapshots for the next step, loaded one step ahead
 #pragma unroll
