@@ -79,6 +79,8 @@ def parse(argv=None):
                     help="only run the serving loop under Poisson arrivals for this many "
                          "seconds of arrivals; prints its own JSON line")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve arrival rate (requests/s)")
+    ap.add_argument("--serve-histories", default="whole", choices=["share", "whole"],
+                    help="the server's snapshot-history admission rule")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then "
                          "WRITE_SIZE; CSV output) into profiles/traffic.json; uses no GPU")
@@ -214,7 +216,7 @@ def main():
         pmc_probe(dev, args)
         return
     if args.serve > 0:
-        print(json.dumps(serve_bench(dev, args.serve, args.rate)), flush=True)
+        print(json.dumps(serve_bench(dev, args.serve, args.rate, args.serve_histories)), flush=True)
         return
     if args.workload == "c2" and args.section is None:
         out = c2_line(dev, args)
@@ -233,7 +235,7 @@ def main():
     elif sec == "plain":
         out = plain_path(dev, args.ref_value)
     else:
-        out = serve_bench(dev, args.serve_check, args.serve_rate)
+        out = serve_bench(dev, args.serve_check, args.serve_rate, args.serve_histories)
     print(json.dumps(out), flush=True)
 
 
@@ -819,7 +821,7 @@ def plain_path(dev, value):
             "what": "per-step eager launches, one shot per launch sequence, no graph"}
 
 
-def serve_bench(dev, seconds, rate):
+def serve_bench(dev, seconds, rate, histories="whole"):
     """Serving loop under Poisson arrivals: requests (grid uniform over
     128x128, 256x128, 512x256, 1024x512; 1-32 shots; 1500-5000 steps by grid
     diagonal) are submitted from a thread at their arrival times, admitted by
@@ -839,7 +841,8 @@ def serve_bench(dev, seconds, rate):
     arrivals = poisson_trace(rate, seconds, seed=4)
     lazy = len(arrivals) > 200
     sampled = set(k * len(arrivals) // 8 for k in range(8))
-    reqs = [] if lazy else [serving_request(i, rng) for i in range(len(arrivals))]
+    pool = {}        # models and noise shared between requests: cheap generation, no GIL stalls
+    reqs = [] if lazy else [serving_request(i, rng, pool=pool) for i in range(len(arrivals))]
     offered_cu = [0]
     # warm-up: one small request of every grid (graph and allocator set-up)
     warm = Server(dev)
@@ -851,13 +854,13 @@ def serve_bench(dev, seconds, rate):
 
     del warm
     torch.cuda.empty_cache()
-    srv = Server(dev)
+    srv = Server(dev, histories=histories)
     t0 = time.perf_counter()
 
     def submitter():
         for i, at in enumerate(arrivals):
             if lazy:
-                reqs.append(serving_request(i, rng))
+                reqs.append(serving_request(i, rng, pool=pool))
             r = reqs[i]
             offered_cu[0] += r.wl.cell_updates_per_gradient()
             d = t0 + at - time.perf_counter()
@@ -908,6 +911,20 @@ def serve_bench(dev, seconds, rate):
         for k in ("er", "se", "traces"):
             x, y = r.result[k].double(), ra.result[k].double()
             worst = max(worst, float((x - y).abs().max() / max(y.abs().max(), 1e-30)))
+    by_grid = {}
+    for r in reqs:
+        if r.done_at is None or r.admitted_at is None:
+            continue
+        by_grid.setdefault(f"{r.wl.NI}x{r.wl.NJ}", []).append(
+            (r.done_at - r.arrival, r.admitted_at - r.arrival, r.wl.cell_updates_per_gradient()))
+    grids = {}
+    for g, v in sorted(by_grid.items(), key=lambda kv: int(kv[0].split("x")[0])):
+        lat = np.array([x[0] for x in v])
+        wait = np.array([x[1] for x in v])
+        grids[g] = {"n": len(v), "p50_s": round(float(np.percentile(lat, 50)), 3),
+                    "p99_s": round(float(np.percentile(lat, 99)), 3),
+                    "queue_p99_s": round(float(np.percentile(wait, 99)), 3),
+                    "mean_gcell_updates": round(float(np.mean([x[2] for x in v])) / 1e9, 2)}
     return {
         "metric": "serving: p50/p99 request latency, sustained gradient Gcell-updates/s",
         "p50_latency_s": round(summ["p50_latency_s"], 3),
@@ -920,11 +937,13 @@ def serve_bench(dev, seconds, rate):
         "peak_admitted_gb": round(summ["peak_admitted_bytes"] / 1e9, 2),
         "refused": summ["refused"], "holds": summ["holds"],
         "partial_histories": summ["partial_histories"],
+        "histories": histories, "history_waits": summ["history_waits"],
         "recompute_share": round(summ["recompute_share"], 4),
         "launches": summ["launches"], "ticks": summ["ticks"],
         "host_ms_per_tick": round(summ["host_ms_per_tick"], 3),
         "admit_host_s": round(summ["admit_host_s"], 3),
         "window_ns": [m0, m1],
+        "by_grid": grids,
         "alone_vs_mix_max_rel_diff": worst}
 
 

@@ -81,17 +81,33 @@ class History:
 
     @property
     def full(self) -> bool:
-        return not self.segments()
+        return self._head()[0] == 0
 
     def first_stored(self) -> int:
         """First step whose snapshot the forward must write to the ring."""
         return max(self.nt - self.R, 0)
 
+    def _head(self) -> tuple:
+        """(segments, steps they cover) of the head in closed form: every
+        segment [s0, e) with s0 > 0 has a non-empty adjoint range; the first,
+        [0, min(C, h)), has one when min(C, h) - 1 >= lowest."""
+        if self.h <= 0:
+            return 0, 0
+        n = -(-self.h // self.C)
+        first = min(self.C, self.h)
+        if first - 1 >= self.lowest:
+            return n, self.h
+        return n - 1, self.h - first
+
+    def n_saves(self) -> int:
+        """len(saves()) without building the list."""
+        return max(-(-self.h // self.C) - 1, 0) if self.h > 0 else 0
+
     def recompute_steps(self) -> int:
-        return sum(e - s0 for s0, e in self.segments())
+        return self._head()[1]
 
     def nbytes(self, slot_bytes: int, state_bytes: int) -> int:
-        return self.R * slot_bytes + len(self.saves()) * state_bytes
+        return self.R * slot_bytes + self.n_saves() * state_bytes
 
 
 def plan(nt: int, slot_bytes: int, state_bytes: int, budget: int, lowest: int = 1) -> History:

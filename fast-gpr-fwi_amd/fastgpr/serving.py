@@ -9,14 +9,18 @@ inversions, continuously batched on one GPU.
   of it, so admission is exact: a request is admitted when a free block
   holds its footprint (``batching.job_bytes``), and nothing else competes
   for that memory.
-* The footprint adapts to the memory free at admission: a gradient keeps
-  its whole snapshot history when that fits in half of the largest free
-  block (all of it when nothing else runs), otherwise the largest snapshot
-  ring that does (``checkpoint.History``: only the steps before the ring are
-  recomputed).  A request whose smallest footprint does not fit waits;
-  one that waited longer than ``max_hold_s`` holds back every younger
-  request until it is admitted, so a stream of small requests cannot starve
-  a large one.
+* The footprint adapts to the memory free at admission.  By default
+  (``histories="whole"``) a gradient keeps its whole snapshot history when
+  that fits in the largest free block; one whose history would fit the
+  arena but not now waits for memory while another gradient runs, for at
+  most ``max_hold_s``, rather than recompute; past that, or when its history
+  exceeds the arena, it takes the largest snapshot ring that fits in half of
+  the largest free block (all of it when nothing else runs)
+  (``checkpoint.History``: only the steps before the ring are recomputed).
+  ``histories="share"`` skips the wait.  A request whose smallest footprint
+  does not fit waits; one that waited longer than ``max_hold_s`` holds back
+  every younger request until it is admitted, so a stream of small requests
+  cannot starve a large one.
 * Admitted requests become ``batching.Job``s; every tick one multi-problem
   launch per (stream, variant, direction) advances every active job by one
   block, so requests that arrive while others run join at the next tick and
@@ -51,6 +55,9 @@ from .solver import to_device
 from .workloads import Workload
 
 
+WAIT = "wait"      # _plan: admit later, with the whole history
+
+
 @dataclass
 class Request:
     rid: int
@@ -59,6 +66,7 @@ class Request:
     obs: np.ndarray | None         # observed traces (S, nt, nrx); None = forward only
     arrival: float = 0.0           # perf_counter() at submission
     done_at: float | None = None
+    admitted_at: float | None = None   # perf_counter() when its job was built
     result: dict | None = None
     cell_updates: int = 0
     tb: tuple | None = None
@@ -66,6 +74,7 @@ class Request:
     recompute_steps: int = 0
     _sizes: tuple | None = None        # (variant, smallest ring, its bytes, whole-history bytes)
     _held_at: int = -1                 # arena version of the last admission that did not fit
+    _waited: bool = False              # put off once to keep its whole history
 
 
 @dataclass
@@ -86,6 +95,7 @@ class Stats:
     refused: int = 0                # requests that failed (invalid, too large, non-finite)
     cancelled: int = 0
     partial_histories: int = 0      # gradients admitted with a snapshot ring shorter than nt
+    history_waits: int = 0          # requests put off to keep a whole history ("whole")
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -101,6 +111,7 @@ class Stats:
                 "admit_host_s": self.admit_host_s,
                 "holds": self.holds, "refused": self.refused, "cancelled": self.cancelled,
                 "partial_histories": self.partial_histories,
+                "history_waits": self.history_waits,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
 
 
@@ -109,11 +120,20 @@ class Server:
 
     ``hbm_budget``: bytes of the arena (default 85 % of the free memory);
     ``variant_for(wl)``: the time-blocking variant of a request without one.
-    ``scheduler`` and ``arena`` may be given (tests)."""
+    ``histories``: "share" -- a gradient keeps its whole snapshot history
+    when it fits in half of the largest free block (all of it when nothing
+    else runs), otherwise the largest ring that does; "whole" -- it keeps
+    its whole history when that fits in the largest free block, and one whose
+    history could fit the arena but not now waits for memory while another
+    gradient runs (for at most ``max_hold_s``) instead of recomputing with a
+    ring.  ``scheduler`` and ``arena`` may be given (tests)."""
 
     def __init__(self, device="cuda:0", hbm_budget: int | None = None, variant_for=None,
                  max_inflight_ticks: int = 64, max_hold_s: float = 2.0, scheduler=None,
-                 arena=None):
+                 arena=None, histories: str = "whole"):
+        if histories not in ("share", "whole"):
+            raise ValueError(f"histories must be 'share' or 'whole', not {histories!r}")
+        self.histories = histories
         self.device = torch.device(device)
         if arena is None:
             if hbm_budget is None:
@@ -175,6 +195,13 @@ class Server:
                           batching.job_bytes(*spec, wl.nt, True, tb))
         _, low, smallest, full = req._sizes
         largest = self.arena.largest_free()
+        if self.histories == "whole":
+            if full <= largest:
+                return tb, History(wl.nt, wl.nt), full, smallest
+            waited = time.perf_counter() - req.arrival > self.max_hold_s
+            if (full <= self.arena.size and not waited
+                    and any(r.obs is not None for r, _, _, _ in self.active)):
+                return tb, WAIT, full, smallest
         share = largest if not self.active else largest // 2
         if full <= share:
             return tb, History(wl.nt, wl.nt), full, smallest
@@ -215,6 +242,12 @@ class Server:
             if smallest > self.arena.size:
                 self._fail(req, f"request needs at least {smallest} bytes, arena {self.arena.size}")
                 continue
+            if hist is WAIT:                 # its whole history will fit once memory frees
+                if not req._waited:
+                    req._waited = True
+                    self.stats.history_waits += 1
+                still.append(req)
+                continue
             off = self.arena.alloc(nbytes)
             if off is None:
                 self.stats.holds += 1
@@ -231,6 +264,7 @@ class Server:
                 self._fail(req, f"could not build the job: {type(e).__name__}: {e}")
                 continue
             req.tb, req.history = tb, hist
+            req.admitted_at = time.perf_counter()
             if hist is not None and not hist.full:
                 self.stats.partial_histories += 1
                 req.recompute_steps = hist.recompute_steps()
@@ -354,12 +388,19 @@ def poisson_trace(rate_hz: float, duration_s: float, seed: int = 4) -> np.ndarra
 
 
 def serving_request(rid: int, rng, grids=((128, 128), (256, 128), (512, 256), (1024, 512)),
-                    shots=(1, 32), steps=(1500, 5000), gradient=True, P=20) -> Request:
+                    shots=(1, 32), steps=(1500, 5000), gradient=True, P=20,
+                    pool: dict | None = None) -> Request:
     """One request of the serving mix: grid drawn uniformly from ``grids``,
     shots uniform in ``shots``, step count interpolated in ``steps`` by the
     grid diagonal (smallest grid -> steps[0], largest -> steps[1]), seeded
     layered + blob model.  The observed traces are seeded noise (the
-    gradient's cost does not depend on them)."""
+    gradient's cost does not depend on them).
+
+    ``pool`` (a dict the caller keeps) makes a long stream of requests cheap
+    to generate: models are built once per (grid, seed mod 8) and the
+    observed traces are a read-only slice of one noise block per (steps,
+    receivers).  The noise is then not drawn from ``rng``, so a pooled
+    stream's requests differ from the pool-less stream's after the first."""
     from .grid import cfl_dt
     from .workloads import _cells, _smooth, layered_blob
     NI, NJ = grids[int(rng.integers(0, len(grids)))]
@@ -368,18 +409,34 @@ def serving_request(rid: int, rng, grids=((128, 128), (256, 128), (512, 256), (1
     span = max(diags) - min(diags)
     frac = 0.0 if span == 0 else (math.hypot(NI, NJ) - min(diags)) / span
     nt = int(round(steps[0] + (steps[1] - steps[0]) * frac))
-    er, se = layered_blob(NI, NJ, seed=int(rng.integers(0, 2**31)), nlayers=6, nblobs=2, P=P)
+    seed = int(rng.integers(0, 2**31))
+    mkey = ("model", NI, NJ, P, seed % 8)
+    if pool is not None and mkey in pool:
+        er, se, er0, se0 = pool[mkey]
+    else:
+        er, se = layered_blob(NI, NJ, seed=seed, nlayers=6, nblobs=2, P=P)
+        er0, se0 = _smooth(er, 5).astype(np.float32), _smooth(se, 5).astype(np.float32)
+        er, se = er.astype(np.float32), se.astype(np.float32)
+        if pool is not None:
+            pool[mkey] = (er, se, er0, se0)
     dx = 0.025
     dt = 0.9 * float(cfl_dt(dx))
     si = np.round(np.linspace(P + 5, NI - P - 6, S)).astype(int)
     nrx = int(max(1, min(64, NI - 2 * P - 10)))
     ri = np.round(np.linspace(P + 5, NI - P - 6, nrx)).astype(int)
     sj, rj = P + 5, NJ - P - 6
-    wl = Workload(f"req{rid}_{NI}x{NJ}_{S}shots", er.astype(np.float32), se.astype(np.float32),
-                  np.ones((NI, NJ), np.float32), _smooth(er, 5).astype(np.float32),
-                  _smooth(se, 5).astype(np.float32), dx, dt, nt, P, 2e8,
+    wl = Workload(f"req{rid}_{NI}x{NJ}_{S}shots", er, se, np.ones((NI, NJ), np.float32), er0, se0,
+                  dx, dt, nt, P, 2e8,
                   _cells([[(int(i), sj)] for i in si]), _cells([[(int(i), rj) for i in ri]] * S))
     obs = None
-    if gradient:
+    if gradient and pool is None:
         obs = rng.standard_normal((S, nt, nrx)).astype(np.float32) * 1e-3
+    elif gradient:
+        nkey = ("noise", nt, nrx, shots[1])
+        if nkey not in pool:
+            block = np.random.default_rng(nt * 1009 + nrx).standard_normal(
+                (shots[1], nt, nrx), dtype=np.float32) * np.float32(1e-3)
+            block.flags.writeable = False
+            pool[nkey] = block
+        obs = pool[nkey][:S]
     return Request(rid, wl, (wl.er0, wl.se0), obs)

@@ -88,7 +88,26 @@ def _blocks(keys: np.ndarray, ents: np.ndarray, nkeys: int, cap: int) -> np.ndar
     return out.astype(np.uint32).view(np.int32).reshape(-1)
 
 
+_TILE_CACHE: dict = {}
+_TILE_CACHE_MAX = 16
+
+
 def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
+    """``_tile_tables`` of the same receivers, grid and variant computed once
+    (a job's forward and adjoint batches, repeated geometries): the last
+    few results are kept; callers copy the arrays and never write them."""
+    rx = np.ascontiguousarray(rx, np.int64)
+    key = (rx.shape, rx.tobytes(), int(NI), int(NJ), tuple(tb))
+    hit = _TILE_CACHE.pop(key, None)
+    if hit is None:
+        hit = _tile_tables(rx, NI, NJ, tb)
+    _TILE_CACHE[key] = hit                       # most recent last
+    while len(_TILE_CACHE) > _TILE_CACHE_MAX:
+        _TILE_CACHE.pop(next(iter(_TILE_CACHE)))
+    return hit
+
+
+def _tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
     """Receiver blocks of the time-blocked kernels (host, numpy).
 
     rx: (S, nrx, 2) int cell indices, i < 0 = dead.  Forward: each live
@@ -104,25 +123,39 @@ def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
     if nrx >= 1 << 20:
         raise ValueError("the time-blocked path holds at most 2^20 - 1 receivers per shot")
     ntiles = ti * tj
-    fk, fv, ak, av = [], [], [], []
-    for s in range(S):
-        for r in range(nrx):
-            i, j = int(rx[s, r, 0]), int(rx[s, r, 1])
-            if i < 0:
-                continue
-            a, b = i // TI, j // TJ
-            fk.append(s * ntiles + a * tj + b)
-            fv.append(((i - a * TI + k) << 26) | ((j - b * TJ + k) << 20) | r)
-            if i >= NI or j >= NJ:
-                continue
-            for a in range(max(0, (i + k - ri) // TI), min(ti - 1, (i + k) // TI) + 1):
-                if not (a * TI - k <= i < a * TI - k + ri):
-                    continue
-                for b in range(max(0, (j + k - 64) // TJ), min(tj - 1, (j + k) // TJ) + 1):
-                    if b * TJ - k <= j < b * TJ - k + 64:
-                        ak.append(s * ntiles + a * tj + b)
-                        av.append(((i - a * TI + k) << 26) | ((j - b * TJ + k) << 20) | r)
-    fk, ak = np.asarray(fk, np.int64), np.asarray(ak, np.int64)
+    # every (shot, receiver) in shot-major, receiver order
+    sh = np.repeat(np.arange(S, dtype=np.int64), nrx)
+    r = np.tile(np.arange(nrx, dtype=np.int64), S)
+    i = np.asarray(rx[..., 0], np.int64).reshape(-1)
+    j = np.asarray(rx[..., 1], np.int64).reshape(-1)
+    live = i >= 0
+    sh, r, i, j = sh[live], r[live], i[live], j[live]
+    a, b = i // TI, j // TJ
+    fk = sh * ntiles + a * tj + b
+    fv = ((i - a * TI + k) << 26) | ((j - b * TJ + k) << 20) | r
+    # adjoint: every region (centre + halo) holding an updated receiver cell;
+    # a region spans fewer than ri // TI + 2 tile rows and 64 // TJ + 2 columns
+    upd = (i < NI) & (j < NJ)
+    su, ru, iu, ju = sh[upd], r[upd], i[upd], j[upd]
+    a_lo = np.maximum(0, (iu + k - ri) // TI)
+    a_hi = np.minimum(ti - 1, (iu + k) // TI)
+    b_lo = np.maximum(0, (ju + k - 64) // TJ)
+    b_hi = np.minimum(tj - 1, (ju + k) // TJ)
+    keys, rs, vals = [], [], []
+    for da in range(ri // TI + 2):
+        aa = a_lo + da
+        oka = (aa <= a_hi) & (aa * TI - k <= iu) & (iu < aa * TI - k + ri)
+        for db in range(64 // TJ + 2):
+            bb = b_lo + db
+            ok = oka & (bb <= b_hi) & (bb * TJ - k <= ju) & (ju < bb * TJ - k + 64)
+            keys.append((su * ntiles + aa * tj + bb)[ok])
+            rs.append(ru[ok])
+            vals.append((((iu - aa * TI + k) << 26) | ((ju - bb * TJ + k) << 20) | ru)[ok])
+    ak = np.concatenate(keys) if keys else np.zeros(0, np.int64)
+    rr = np.concatenate(rs) if rs else np.zeros(0, np.int64)
+    av = np.concatenate(vals) if vals else np.zeros(0, np.int64)
+    order = np.lexsort((rr, ak))          # per region, in receiver order
+    ak, av = ak[order], av[order]
     most = max(int(np.bincount(fk, minlength=1).max()) if len(fk) else 0,
                int(np.bincount(ak, minlength=1).max()) if len(ak) else 0)
     if most > TB_MAX_LIST:
@@ -130,8 +163,8 @@ def tile_tables(rx: np.ndarray, NI: int, NJ: int, tb) -> dict:
                          f"(at most {TB_MAX_LIST}); use smaller regions or the per-step kernels")
     cap = 64 if most <= 64 else 128
     return dict(TI=TI, TJ=TJ, ti=ti, tj=tj, cap=cap,
-                fwd=_blocks(fk, np.asarray(fv, np.int64), S * ntiles, cap),
-                adj=_blocks(ak, np.asarray(av, np.int64), S * ntiles, cap))
+                fwd=_blocks(fk, fv, S * ntiles, cap),
+                adj=_blocks(ak, av, S * ntiles, cap))
 
 
 class Batch:

@@ -65,9 +65,10 @@ class FakeScheduler:
         pass
 
 
-def _server(arena_bytes, ticks=None, bad=(), nonfinite=(), max_hold_s=2.0):
+def _server(arena_bytes, ticks=None, bad=(), nonfinite=(), max_hold_s=2.0, histories="share"):
     arena = batching.Arena("cpu", arena_bytes)
-    srv = Server("cpu", scheduler=FakeScheduler(), arena=arena, max_hold_s=max_hold_s)
+    srv = Server("cpu", scheduler=FakeScheduler(), arena=arena, max_hold_s=max_hold_s,
+                 histories=histories)
     built = []
 
     def make(req, tb, hist, region):
@@ -127,6 +128,45 @@ def test_admission_is_exact_and_histories_adapt():
     assert srv.stats.partial_histories == 1
     st = srv.run()
     assert st["completed"] == 2 and st["recompute_share"] > 0
+
+
+def test_whole_histories_wait_for_memory_instead_of_recomputing():
+    """histories="whole": the second gradient, whose whole history fits the
+    arena but not beside the first, waits for the first to finish and then
+    keeps its whole history -- no ring, no recompute."""
+    reqs = _reqs(2, grids=((96, 64),), shots=(4, 4), steps=(2000, 2000))
+    full = _bytes(reqs[0])
+    srv, built = _server(int(1.5 * full), ticks={0: 3, 1: 3}, histories="whole")
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    assert len(srv.active) == 1 and srv.stats.history_waits >= 1
+    st = srv.run()
+    assert st["completed"] == 2 and st["recompute_share"] == 0
+    assert [h.full for _, h in built] == [True, True]
+    assert st["history_waits"] >= 1 and st["partial_histories"] == 0
+
+
+def test_whole_histories_take_a_ring_after_the_age_bound_or_when_too_large():
+    """histories="whole": a gradient that waited longer than max_hold_s is
+    admitted with a ring beside the running one; one whose whole history
+    exceeds the arena never waits for it."""
+    reqs = _reqs(2, grids=((96, 64),), shots=(4, 4), steps=(2000, 2000))
+    full = _bytes(reqs[0])
+    srv, built = _server(int(1.5 * full), ticks={0: 50, 1: 3}, histories="whole",
+                         max_hold_s=0.0)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    assert len(srv.active) == 2 and built[0][1].full and not built[1][1].full
+    big = _reqs(1, grids=((96, 64),), shots=(4, 4), steps=(2000, 2000))[0]
+    srv, built = _server(int(0.6 * _bytes(big)), histories="whole")
+    srv.submit(big)
+    st = srv.run()
+    assert st["completed"] == 1 and not built[0][1].full and st["history_waits"] == 0
+    with pytest.raises(ValueError):
+        Server("cpu", scheduler=FakeScheduler(), arena=batching.Arena("cpu", 1 << 20),
+               histories="most")
 
 
 def test_held_request_blocks_younger_ones_after_its_age_bound():
