@@ -79,7 +79,7 @@ def parse(argv=None):
                     help="only run the serving loop under Poisson arrivals for this many "
                          "seconds of arrivals; prints its own JSON line")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve arrival rate (requests/s)")
-    ap.add_argument("--serve-histories", default="whole", choices=["share", "whole"],
+    ap.add_argument("--serve-histories", default="strict", choices=["share", "whole", "strict"],
                     help="the server's snapshot-history admission rule")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then "
@@ -821,7 +821,7 @@ def plain_path(dev, value):
             "what": "per-step eager launches, one shot per launch sequence, no graph"}
 
 
-def serve_bench(dev, seconds, rate, histories="whole"):
+def serve_bench(dev, seconds, rate, histories="strict"):
     """Serving loop under Poisson arrivals: requests (grid uniform over
     128x128, 256x128, 512x256, 1024x512; 1-32 shots; 1500-5000 steps by grid
     diagonal) are submitted from a thread at their arrival times, admitted by

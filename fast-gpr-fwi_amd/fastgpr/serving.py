@@ -10,17 +10,19 @@ inversions, continuously batched on one GPU.
   holds its footprint (``batching.job_bytes``), and nothing else competes
   for that memory.
 * The footprint adapts to the memory free at admission.  By default
-  (``histories="whole"``) a gradient keeps its whole snapshot history when
+  (``histories="strict"``) a gradient keeps its whole snapshot history when
   that fits in the largest free block; one whose history would fit the
-  arena but not now waits for memory while another gradient runs, for at
-  most ``max_hold_s``, rather than recompute; past that, or when its history
-  exceeds the arena, it takes the largest snapshot ring that fits in half of
-  the largest free block (all of it when nothing else runs)
-  (``checkpoint.History``: only the steps before the ring are recomputed).
-  ``histories="share"`` skips the wait.  A request whose smallest footprint
-  does not fit waits; one that waited longer than ``max_hold_s`` holds back
-  every younger request until it is admitted, so a stream of small requests
-  cannot starve a large one.
+  arena but not now waits for memory while another gradient runs rather
+  than recompute, and past ``max_hold_s`` holds back every younger request
+  until it fits.  Only a gradient whose history exceeds the arena takes a
+  snapshot ring: the largest that fits in half of the largest free block
+  (all of it when nothing else runs) (``checkpoint.History``: only the steps
+  before the ring are recomputed).  ``histories="whole"`` takes the ring
+  once ``max_hold_s`` has passed instead; ``"share"`` never waits for a
+  whole history.  A request whose smallest footprint does not fit waits;
+  one that waited longer than ``max_hold_s`` holds back every younger
+  request until it is admitted, so a stream of small requests cannot starve
+  a large one.
 * Admitted requests become ``batching.Job``s; every tick one multi-problem
   launch per (stream, variant, direction) advances every active job by one
   block, so requests that arrive while others run join at the next tick and
@@ -95,7 +97,7 @@ class Stats:
     refused: int = 0                # requests that failed (invalid, too large, non-finite)
     cancelled: int = 0
     partial_histories: int = 0      # gradients admitted with a snapshot ring shorter than nt
-    history_waits: int = 0          # requests put off to keep a whole history ("whole")
+    history_waits: int = 0          # requests put off to keep a whole history ("whole", "strict")
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -126,13 +128,16 @@ class Server:
     its whole history when that fits in the largest free block, and one whose
     history could fit the arena but not now waits for memory while another
     gradient runs (for at most ``max_hold_s``) instead of recomputing with a
-    ring.  ``scheduler`` and ``arena`` may be given (tests)."""
+    ring; "strict" -- as "whole", but past ``max_hold_s`` it holds back every
+    younger request until its whole history fits, and only a gradient whose
+    history exceeds the arena takes a ring.  ``scheduler`` and ``arena`` may
+    be given (tests)."""
 
     def __init__(self, device="cuda:0", hbm_budget: int | None = None, variant_for=None,
                  max_inflight_ticks: int = 64, max_hold_s: float = 2.0, scheduler=None,
-                 arena=None, histories: str = "whole"):
-        if histories not in ("share", "whole"):
-            raise ValueError(f"histories must be 'share' or 'whole', not {histories!r}")
+                 arena=None, histories: str = "strict"):
+        if histories not in ("share", "whole", "strict"):
+            raise ValueError(f"histories must be 'share', 'whole' or 'strict', not {histories!r}")
         self.histories = histories
         self.device = torch.device(device)
         if arena is None:
@@ -195,10 +200,11 @@ class Server:
                           batching.job_bytes(*spec, wl.nt, True, tb))
         _, low, smallest, full = req._sizes
         largest = self.arena.largest_free()
-        if self.histories == "whole":
+        if self.histories in ("whole", "strict"):
             if full <= largest:
                 return tb, History(wl.nt, wl.nt), full, smallest
-            waited = time.perf_counter() - req.arrival > self.max_hold_s
+            waited = (self.histories == "whole"
+                      and time.perf_counter() - req.arrival > self.max_hold_s)
             if (full <= self.arena.size and not waited
                     and any(r.obs is not None for r, _, _, _ in self.active)):
                 return tb, WAIT, full, smallest
@@ -247,6 +253,8 @@ class Server:
                     req._waited = True
                     self.stats.history_waits += 1
                 still.append(req)
+                if now - req.arrival > self.max_hold_s:
+                    blocked = True           # "strict": memory drains for it
                 continue
             off = self.arena.alloc(nbytes)
             if off is None:

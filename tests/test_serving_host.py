@@ -169,6 +169,28 @@ def test_whole_histories_take_a_ring_after_the_age_bound_or_when_too_large():
                histories="most")
 
 
+def test_strict_histories_hold_back_younger_requests_for_a_whole_history():
+    """histories="strict": a gradient that waited past max_hold_s for its
+    whole history does not take a ring; younger requests wait behind it
+    until it fits, and nothing is recomputed."""
+    big = _reqs(2, grids=((96, 64),), shots=(4, 4), steps=(2000, 2000))
+    small = _reqs(1, seed=9, grids=((64, 48),), shots=(1, 1), steps=(100, 100))[0]
+    small.rid = 2
+    full = _bytes(big[0])
+    srv, built = _server(int(1.5 * full), ticks={0: 4, 1: 3, 2: 1}, histories="strict",
+                         max_hold_s=0.0)
+    srv.submit(big[0])
+    srv.submit(big[1])
+    srv.step()
+    assert [r.rid for r, _, _, _ in srv.active] == [0]
+    srv.submit(small)
+    srv.step()                                   # big[1] is past its age bound: small waits
+    assert [r.rid for r, _, _, _ in srv.active] == [0] and len(srv.waiting) == 2
+    st = srv.run()
+    assert st["completed"] == 3 and st["recompute_share"] == 0
+    assert [rid for rid, _ in built] == [0, 1, 2] and all(h.full for _, h in built[:2])
+
+
 def test_held_request_blocks_younger_ones_after_its_age_bound():
     """A request whose smallest footprint does not fit waits (a hold); while
     it is young smaller ones pass it, once it is older than max_hold_s
