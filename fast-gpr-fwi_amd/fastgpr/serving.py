@@ -15,11 +15,11 @@ inversions, continuously batched on one GPU.
   arena but not now waits for memory while another gradient runs rather
   than recompute, and past ``max_hold_s`` holds back every younger request
   until it fits.  Only a gradient whose history exceeds the arena takes a
-  snapshot ring: the largest that fits in half of the largest free block
-  (all of it when nothing else runs) (``checkpoint.History``: only the steps
-  before the ring are recomputed).  ``histories="whole"`` takes the ring
-  once ``max_hold_s`` has passed instead; ``"share"`` never waits for a
-  whole history.  A request whose smallest footprint does not fit waits;
+  snapshot ring: the largest that fits in the largest free block
+  (``checkpoint.History``: only the steps before the ring are recomputed).
+  ``histories="whole"`` takes a ring once ``max_hold_s`` has passed instead,
+  and ``"share"`` never waits for a whole history; both size a ring to half
+  of the largest free block while other requests run.  A request whose smallest footprint does not fit waits;
   one that waited longer than ``max_hold_s`` holds back every younger
   request until it is admitted, so a stream of small requests cannot starve
   a large one.
@@ -130,8 +130,8 @@ class Server:
     gradient runs (for at most ``max_hold_s``) instead of recomputing with a
     ring; "strict" -- as "whole", but past ``max_hold_s`` it holds back every
     younger request until its whole history fits, and only a gradient whose
-    history exceeds the arena takes a ring.  ``scheduler`` and ``arena`` may
-    be given (tests)."""
+    history exceeds the arena takes a ring, sized to the largest free block.
+    ``scheduler`` and ``arena`` may be given (tests)."""
 
     def __init__(self, device="cuda:0", hbm_budget: int | None = None, variant_for=None,
                  max_inflight_ticks: int = 64, max_hold_s: float = 2.0, scheduler=None,
@@ -208,7 +208,9 @@ class Server:
             if (full <= self.arena.size and not waited
                     and any(r.obs is not None for r, _, _, _ in self.active)):
                 return tb, WAIT, full, smallest
-        share = largest if not self.active else largest // 2
+        # "strict": a history larger than the arena takes the whole largest
+        # free block (the fewer steps recomputed); otherwise half when others run
+        share = largest if (not self.active or self.histories == "strict") else largest // 2
         if full <= share:
             return tb, History(wl.nt, wl.nt), full, smallest
         try:

@@ -191,6 +191,30 @@ def test_strict_histories_hold_back_younger_requests_for_a_whole_history():
     assert [rid for rid, _ in built] == [0, 1, 2] and all(h.full for _, h in built[:2])
 
 
+def test_strict_sizes_an_oversize_ring_to_the_largest_free_block():
+    """A gradient whose whole history exceeds the arena, admitted while
+    another runs: "strict" sizes its ring to the largest free block, "whole"
+    to half of it (more recompute)."""
+    small = _reqs(1, grids=((64, 48),), shots=(1, 1), steps=(100, 100))[0]
+    big = _reqs(1, seed=5, grids=((96, 64),), shots=(4, 4), steps=(2000, 2000))[0]
+    big.rid = 1
+    arena = int(0.8 * _bytes(big))
+    used = {}
+    for rule in ("strict", "whole"):
+        srv, built = _server(arena, ticks={0: 50, 1: 3}, histories=rule)
+        srv.submit(small)
+        srv.step()
+        free = srv.arena.largest_free()
+        srv.submit(big)
+        srv.step()
+        (rid, hist), = [b for b in built if b[0] == 1]
+        assert not hist.full
+        used[rule] = _bytes(big, hist.R)
+        assert used[rule] <= free
+        small.result = small.done_at = None
+    assert used["strict"] > free // 2 >= used["whole"] * 0.99
+
+
 def test_held_request_blocks_younger_ones_after_its_age_bound():
     """A request whose smallest footprint does not fit waits (a hold); while
     it is young smaller ones pass it, once it is older than max_hold_s
