@@ -19,10 +19,10 @@ inversions, continuously batched on one GPU.
   (``checkpoint.History``: only the steps before the ring are recomputed).
   ``histories="whole"`` takes a ring once ``max_hold_s`` has passed instead,
   and ``"share"`` never waits for a whole history; both size a ring to half
-  of the largest free block while other requests run.  A request whose smallest footprint does not fit waits;
-  one that waited longer than ``max_hold_s`` holds back every younger
-  request until it is admitted, so a stream of small requests cannot starve
-  a large one.
+  of the largest free block while other requests run.  A request whose
+  smallest footprint does not fit waits; one that waited longer than
+  ``max_hold_s`` holds back every younger request until it is admitted, so
+  a stream of small requests cannot starve a large one.
 * Admitted requests become ``batching.Job``s; every tick one multi-problem
   launch per (stream, variant, direction) advances every active job by one
   block, so requests that arrive while others run join at the next tick and
