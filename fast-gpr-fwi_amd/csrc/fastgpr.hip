@@ -1085,9 +1085,13 @@ __device__ __forceinline__ void tb_body(const TBArgs& a, const TBArgs* ap, const
     // (a ring of snap_mod slots, or n - snap_base), never by a division per step.
     const int64_t snap_slot = (int64_t)d.S * NI * pitch;
     float* const snap_wave = d.snaps ? d.snaps + ((int64_t)s * NI + gi0 + r0) * pitch : nullptr;
-    auto slot_of = [&](int n) -> int { return d.snap_mod ? n % d.snap_mod : n - d.snap_base; };
-    auto slot_next = [&](int sl) -> int { return (d.snap_mod && sl + 1 == d.snap_mod) ? 0 : sl + 1; };
-    auto slot_prev = [&](int sl) -> int { return (d.snap_mod && sl == 0) ? d.snap_mod - 1 : sl - 1; };
+    // Ring size and base read once: in a multi-problem launch `d` lives in the
+    // launch table in global memory, and a read inside the step loop would be
+    // a vector load whose wait also drains the step's snapshot stores.
+    const int snap_mod = d.snap_mod, snap_base = d.snap_base;
+    auto slot_of = [&](int n) -> int { return snap_mod ? n % snap_mod : n - snap_base; };
+    auto slot_next = [&](int sl) -> int { return (snap_mod && sl + 1 == snap_mod) ? 0 : sl + 1; };
+    auto slot_prev = [&](int sl) -> int { return (snap_mod && sl == 0) ? snap_mod - 1 : sl - 1; };
     auto snap_ref = [&](int sl, int m) -> __attribute__((address_space(1))) float& {
         return gmem(snap_wave + (int64_t)sl * snap_slot + m * pitch)[gj];
     };
