@@ -92,6 +92,11 @@ def parse(argv=None):
                          "run starts one process per section)")
     ap.add_argument("--ref-value", type=float, default=None,
                     help="--section plain: the headline value the plain path is compared with")
+    ap.add_argument("--envelope", action="store_true",
+                    help="autotuner envelope: on cross-hole grids 200x100..1600x800 with 1..32 "
+                         "shots, time every variant graph-replayed (the tuner) and launched one "
+                         "by one, and the full gradient with each of the fastest three pinned; "
+                         "one JSON line per grid")
     ap.add_argument("--kernel-busy", nargs=3, metavar=("DB", "T0_NS", "T1_NS"),
                     help="read a rocprofv3 kernel-trace database and print how much of the "
                          "monotonic window [T0, T1] (the serving line's window_ns) had a kernel "
@@ -206,7 +211,7 @@ def main():
         sys.exit(2)
 
     if args.section is None and args.workload == "c3" and not args.quick and not args.pmc_probe \
-            and args.serve <= 0:
+            and args.serve <= 0 and not args.envelope:
         sys.exit(run_sections(args))
 
     import torch
@@ -214,6 +219,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.pmc_probe:
         pmc_probe(dev, args)
+        return
+    if args.envelope:
+        tuner_envelope(dev)
         return
     if args.serve > 0:
         print(json.dumps(serve_bench(dev, args.serve, args.rate, args.serve_histories)), flush=True)
@@ -781,6 +789,86 @@ def pmc_probe(dev, args):
     torch.cuda.synchronize()
     print(json.dumps({"probe": "done", "c3": roof["workload_key"],
                       "c2": f"{wl.name}|k_tb<64,1024,6,true>"}), flush=True)
+
+
+# Grids and shot counts of the autotuner envelope: (NI, NJ, S, steps).  The
+# 1600 x 800 32-shot batch is left out: its history (320 GB) does not fit.
+ENVELOPE = [(NI, NJ, S, nt) for NI, NJ, nt in ((200, 100, 3000), (400, 200, 3000),
+                                              (800, 400, 1500), (1600, 800, 1500))
+            for S in (1, 4, 13, 32) if (NI, S) != (1600, 32)]
+
+
+def tuner_envelope(dev, cases=ENVELOPE, grad_reps=3):
+    """For each grid: every variant's time per step as the tuner takes it
+    (graph-replayed blocks) and launched one by one, the picks of both, and
+    the gradient rate (compute + backward, two timings of ``grad_reps``)
+    with each of the three fastest variants of either timing pinned."""
+    import torch
+    import fastgpr
+    from fastgpr import autotune
+    from fastgpr.workloads import crosshole
+    rows = []
+    for NI, NJ, S, nt in cases:
+        wl = crosshole(NI, NJ, S, nt)
+        er_t, se_t, mr = _tensors(wl, dev, (wl.er, wl.se))
+        er0, se0, _ = _tensors(wl, dev, (wl.er0, wl.se0))
+        with torch.no_grad():
+            obs = _compute(wl, dev, er_t, se_t, mr, time_block=(64, 1024, 6)).detach().clone()
+        G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window, ['hertzian', 'z'],
+                                   None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
+                                   er0, se0, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
+                                   False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx),
+                                   time_block="auto", shot_streams=3)
+        reps = autotune.timing_reps(G)
+        times = {"graph": {}, "eager": {}}
+        for mode in times:
+            for tb in autotune.CANDIDATES:
+                try:
+                    _native_tiles(G, tb)
+                except (RuntimeError, ValueError):
+                    continue
+                times[mode][tb] = 1e6 * autotune.time_variant(G, er0, se0, mr, tb, reps=reps,
+                                                              graph=mode == "graph")
+        top = set()
+        for t in times.values():
+            top |= set(sorted(t, key=t.get)[:3])
+        er = er0.clone().requires_grad_(True)
+        se = se0.clone().requires_grad_(True)
+
+        def rate(tb):
+            def step():
+                er.grad = None
+                se.grad = None
+                pred = _compute(wl, dev, er, se, mr, time_block=tb, shot_streams=3)
+                (0.5 * ((pred - obs) ** 2).sum()).backward()
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(grad_reps):
+                step()
+            torch.cuda.synchronize()
+            return wl.cell_updates_per_gradient() / ((time.perf_counter() - t0) / grad_reps) / 1e9
+        rates = {tb: [round(rate(tb), 1) for _ in range(2)] for tb in sorted(top)}
+        best = max(rates, key=lambda tb: sum(rates[tb]))
+        picks = {m: min(t, key=t.get) for m, t in times.items()}
+        row = {"grid": f"{NI}x{NJ}", "shots": S, "steps": nt,
+               "pick": list(picks["graph"]), "eager_pick": list(picks["eager"]),
+               "best_pinned": list(best),
+               "pick_vs_best": round(sum(rates[picks["graph"]]) / sum(rates[best]), 4)
+               if picks["graph"] in rates else None,
+               "variants": {",".join(map(str, tb)): {
+                   "graph_us_per_step": round(times["graph"][tb], 3),
+                   "eager_us_per_step": round(times["eager"][tb], 3),
+                   "gradient_gcell_updates_per_s": rates[tb]} for tb in sorted(top)}}
+        print(json.dumps(row), flush=True)
+        rows.append(row)
+        fastgpr.solver.release_graphs()
+    return rows
+
+
+def _native_tiles(G, tb):
+    from fastgpr import _native
+    _native.tb_tiles(G.NI, G.NJ, *tb)
 
 
 def plain_path(dev, value):

@@ -1611,6 +1611,10 @@ struct GraphEntry {
 
 std::mutex g_graph_mu;
 std::map<GraphKey, GraphEntry> g_graphs;
+// Graphs of fgpr_profile_kernel's timed launches: a cache of their own, so
+// timing (the autotuner) never evicts a sweep's graphs.
+std::map<GraphKey, GraphEntry> g_prof_graphs;
+constexpr size_t kProfGraphLimit = 8;
 uint64_t g_graph_clock = 0;
 size_t g_graph_limit = 64;
 
@@ -1642,47 +1646,37 @@ void destroy_entry(GraphEntry& e) {
 }
 
 // Least recently launched entries go first; `keep` is never evicted.
-void evict_over_limit(const GraphEntry* keep) {
-    while (g_graphs.size() > g_graph_limit) {
-        auto victim = g_graphs.end();
-        for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+void evict_over_limit(std::map<GraphKey, GraphEntry>& cache, size_t limit, const GraphEntry* keep) {
+    while (cache.size() > limit) {
+        auto victim = cache.end();
+        for (auto it = cache.begin(); it != cache.end(); ++it)
             if (&it->second != keep && it->second.evictable &&
-                (victim == g_graphs.end() || it->second.used < victim->second.used))
+                (victim == cache.end() || it->second.used < victim->second.used))
                 victim = it;
-        if (victim == g_graphs.end()) return;
+        if (victim == cache.end()) return;
         destroy_entry(victim->second);
-        g_graphs.erase(victim);
+        cache.erase(victim);
     }
 }
 
-// kind 0 = forward (counter advances by +chunk), 1 = reference adjoint,
-// 2 = exact adjoint (both -chunk).  Called with g_graph_mu held.
-int capture_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* out) {
+// Capture what `body` launches on a private stream into one executable
+// graph.  Called with g_graph_mu held.
+template <typename Body>
+int capture_body(const Body& body, hipGraphExec_t* out) {
     hipStream_t cs;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    const Dev d = to_dev(p);
-    const int dir = kind == 0 ? 1 : -1;
     hipGraph_t g;
-    int rc = FGPR_OK;
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipStreamDestroy(cs);
         return fail(FGPR_EHIP, "hipStreamBeginCapture failed");
     }
-    if (kind == 2) {
-        for (int k = 0; k < chunk && rc == FGPR_OK; ++k) rc = launch_exact_step(d, p, k, dir, cs);
-    } else if (tb_enabled(p)) {
-        // chunk is a multiple of 2*tb_k: an even number of ping-pong launches,
-        // so every replay starts from the primary buffers.
-        for (int c = 0; c * p.tb_k < chunk && rc == FGPR_OK; ++c)
-            rc = launch_tb(p, d, kind == 1, c & 1, c * p.tb_k, dir, p.tb_k, cs);
-    } else {
-        for (int k = 0; k < chunk && rc == FGPR_OK; ++k)
-            rc = kind == 0 ? launch_forward_step(d, p, k, dir, cs) : launch_adjoint_step(d, p, k, dir, cs);
-    }
-    hipLaunchKernelGGL(k_advance_ctr, dim3(1), dim3(64), 0, cs, p.ctr, dir * chunk);
+    const int rc = body(cs);
     hipError_t ec = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
-    if (rc != FGPR_OK) return rc;
+    if (rc != FGPR_OK) {
+        if (ec == hipSuccess) (void)hipGraphDestroy(g);
+        return rc;
+    }
     if (ec != hipSuccess) return fail(FGPR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
     hipGraphExec_t ex;
     ec = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -1692,21 +1686,22 @@ int capture_graph(const fgpr_problem& p, int kind, int chunk, hipGraphExec_t* ou
     return FGPR_OK;
 }
 
-// Launch the graph of (p, kind, chunk) `reps` times on st, capturing it on
+// Launch the cached graph of `key` `reps` times on st, capturing `body` on
 // first use; records the entry's completion event after the launches.
-int launch_graph(const fgpr_problem& p, int kind, int chunk, int reps, hipStream_t st) {
+template <typename Body>
+int launch_cached(std::map<GraphKey, GraphEntry>& cache, size_t limit, GraphKey key,
+                  const Body& body, int reps, hipStream_t st) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    GraphKey key = make_key(p, kind, chunk);
-    auto it = g_graphs.find(key);
-    if (it == g_graphs.end()) {
+    auto it = cache.find(key);
+    if (it == cache.end()) {
         GraphEntry e;
-        int rc = capture_graph(p, kind, chunk, &e.ex);
+        int rc = capture_body(body, &e.ex);
         if (rc) return rc;
         if (hipEventCreateWithFlags(&e.done, hipEventDisableTiming) != hipSuccess) {
             (void)hipGraphExecDestroy(e.ex);
             return fail(FGPR_EHIP, "hipEventCreateWithFlags failed");
         }
-        it = g_graphs.emplace(std::move(key), e).first;
+        it = cache.emplace(std::move(key), e).first;
     }
     GraphEntry& e = it->second;
     for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(e.ex, st));
@@ -1718,8 +1713,33 @@ int launch_graph(const fgpr_problem& p, int kind, int chunk, int reps, hipStream
         e.evictable = false;          // launched into the caller's capture: kept until released
     }
     e.used = ++g_graph_clock;
-    evict_over_limit(&e);
+    evict_over_limit(cache, limit, &e);
     return FGPR_OK;
+}
+
+// The sweep graphs: kind 0 = forward (counter advances by +chunk),
+// 1 = reference adjoint, 2 = exact adjoint (both -chunk); `chunk` steps from
+// the step the descriptor's counter holds.
+int launch_graph(const fgpr_problem& p, int kind, int chunk, int reps, hipStream_t st) {
+    auto body = [&](hipStream_t cs) {
+        const Dev d = to_dev(p);
+        const int dir = kind == 0 ? 1 : -1;
+        int rc = FGPR_OK;
+        if (kind == 2) {
+            for (int k = 0; k < chunk && rc == FGPR_OK; ++k) rc = launch_exact_step(d, p, k, dir, cs);
+        } else if (tb_enabled(p)) {
+            // chunk is a multiple of 2*tb_k: an even number of ping-pong launches,
+            // so every replay starts from the primary buffers.
+            for (int c = 0; c * p.tb_k < chunk && rc == FGPR_OK; ++c)
+                rc = launch_tb(p, d, kind == 1, c & 1, c * p.tb_k, dir, p.tb_k, cs);
+        } else {
+            for (int k = 0; k < chunk && rc == FGPR_OK; ++k)
+                rc = kind == 0 ? launch_forward_step(d, p, k, dir, cs) : launch_adjoint_step(d, p, k, dir, cs);
+        }
+        if (rc == FGPR_OK) hipLaunchKernelGGL(k_advance_ctr, dim3(1), dim3(64), 0, cs, p.ctr, dir * chunk);
+        return rc;
+    };
+    return launch_cached(g_graphs, g_graph_limit, make_key(p, kind, chunk), body, reps, st);
 }
 
 int check_problem(const fgpr_problem* p) {
@@ -2122,52 +2142,74 @@ int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void
     int rc = check_problem(p);
     if (rc) return rc;
     if (n < 0 || n >= p->nt || reps < 1) return fail(FGPR_EINVAL, "bad step or repetition count");
+    const bool graph = (kernel & FGPR_PROFILE_GRAPH) != 0;
+    kernel &= ~FGPR_PROFILE_GRAPH;
     fgpr_problem q = *p;
     q.ctr = nullptr;
     const Dev d = to_dev(q);
     const dim3 grid = cell_grid(q), blk(kBlockJ, kBlockI, 1);
     hipStream_t st = (hipStream_t)stream;
     const int ninj = q.S * q.nrx;
-    for (int r = 0; r < reps; ++r) {
-        switch (kernel) {
-            case FGPR_KERNEL_FWD_H:
-                if (!q.traces && q.nrx > 0) return fail(FGPR_EINVAL, "null traces");
-                hipLaunchKernelGGL(k_h_phase<true>, grid, blk, 0, st, d, n, 1);
-                break;
-            case FGPR_KERNEL_FWD_E:
-                if (q.nsrc > 0 && (!q.src || !q.srcscale)) return fail(FGPR_EINVAL, "null source arrays");
-                if (q.snaps)
-                    hipLaunchKernelGGL((k_e_phase<true, true, false>), grid, blk, 0, st, d, n, 1);
-                else
-                    hipLaunchKernelGGL((k_e_phase<true, false, false>), grid, blk, 0, st, d, n, 1);
-                break;
-            case FGPR_KERNEL_ADJ_INJECT:
-                if (ninj > 0 && q.resid)
-                    hipLaunchKernelGGL(k_adj_inject, dim3((ninj + 255) / 256), dim3(256), 0, st, d, n, 1);
-                break;
-            case FGPR_KERNEL_ADJ_E:
-                if (n < 1 || !q.snaps || !q.grad_eps || !q.grad_sig)
-                    return fail(FGPR_EINVAL, "adjoint E kernel needs n >= 1, snaps and gradients");
-                hipLaunchKernelGGL((k_e_phase<false, false, true>), grid, blk, 0, st, d, n, 1);
-                break;
-            case FGPR_KERNEL_ADJ_H:
-                hipLaunchKernelGGL(k_h_phase<false>, grid, blk, 0, st, d, n, 1);
-                break;
-            case FGPR_KERNEL_TB_FWD:
-            case FGPR_KERNEL_TB_ADJ: {
-                const bool adj = kernel == FGPR_KERNEL_TB_ADJ;
-                if ((rc = tb_check(q, adj))) return rc;
-                if (adj && (n < q.tb_k || !q.snaps)) return fail(FGPR_EINVAL, "adjoint block needs n >= tb_k and snaps");
-                if (!adj && n + q.tb_k > q.nt) return fail(FGPR_EINVAL, "forward block runs past nt");
-                if ((rc = launch_tb(q, d, adj, r & 1, n, 1, q.tb_k, st))) return rc;
-                break;
-            }
-            default:
-                return fail(FGPR_EINVAL, "unknown kernel id");
+    switch (kernel) {        // argument checks, before anything is launched or captured
+        case FGPR_KERNEL_FWD_H:
+            if (!q.traces && q.nrx > 0) return fail(FGPR_EINVAL, "null traces");
+            break;
+        case FGPR_KERNEL_FWD_E:
+            if (q.nsrc > 0 && (!q.src || !q.srcscale)) return fail(FGPR_EINVAL, "null source arrays");
+            break;
+        case FGPR_KERNEL_ADJ_INJECT:
+        case FGPR_KERNEL_ADJ_H:
+            break;
+        case FGPR_KERNEL_ADJ_E:
+            if (n < 1 || !q.snaps || !q.grad_eps || !q.grad_sig)
+                return fail(FGPR_EINVAL, "adjoint E kernel needs n >= 1, snaps and gradients");
+            break;
+        case FGPR_KERNEL_TB_FWD:
+        case FGPR_KERNEL_TB_ADJ: {
+            const bool adj = kernel == FGPR_KERNEL_TB_ADJ;
+            if ((rc = tb_check(q, adj))) return rc;
+            if (adj && (n < q.tb_k || !q.snaps)) return fail(FGPR_EINVAL, "adjoint block needs n >= tb_k and snaps");
+            if (!adj && n + q.tb_k > q.nt) return fail(FGPR_EINVAL, "forward block runs past nt");
+            break;
         }
+        default:
+            return fail(FGPR_EINVAL, "unknown kernel id");
     }
-    HIP_TRY(hipGetLastError());
-    return FGPR_OK;
+    auto body = [&](hipStream_t s) {
+        for (int r = 0; r < reps; ++r) {
+            switch (kernel) {
+                case FGPR_KERNEL_FWD_H:
+                    hipLaunchKernelGGL(k_h_phase<true>, grid, blk, 0, s, d, n, 1);
+                    break;
+                case FGPR_KERNEL_FWD_E:
+                    if (q.snaps)
+                        hipLaunchKernelGGL((k_e_phase<true, true, false>), grid, blk, 0, s, d, n, 1);
+                    else
+                        hipLaunchKernelGGL((k_e_phase<true, false, false>), grid, blk, 0, s, d, n, 1);
+                    break;
+                case FGPR_KERNEL_ADJ_INJECT:
+                    if (ninj > 0 && q.resid)
+                        hipLaunchKernelGGL(k_adj_inject, dim3((ninj + 255) / 256), dim3(256), 0, s, d, n, 1);
+                    break;
+                case FGPR_KERNEL_ADJ_E:
+                    hipLaunchKernelGGL((k_e_phase<false, false, true>), grid, blk, 0, s, d, n, 1);
+                    break;
+                case FGPR_KERNEL_ADJ_H:
+                    hipLaunchKernelGGL(k_h_phase<false>, grid, blk, 0, s, d, n, 1);
+                    break;
+                default: {       // the time-blocked blocks, checked above
+                    const int rc2 = launch_tb(q, d, kernel == FGPR_KERNEL_TB_ADJ, r & 1, n, 1, q.tb_k, s);
+                    if (rc2) return rc2;
+                }
+            }
+        }
+        HIP_TRY(hipGetLastError());
+        return FGPR_OK;
+    };
+    if (!graph) return body(st);
+    GraphKey key = make_key(q, 100 + kernel, reps);
+    key.v.push_back(n);
+    return launch_cached(g_prof_graphs, kProfGraphLimit, std::move(key), body, 1, st);
 }
 
 int fgpr_scale(const double* g, int64_t n, double scale, float* out, void* stream) {
@@ -2388,6 +2430,8 @@ int fgpr_release_graphs(void) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
     for (auto& kv : g_graphs) destroy_entry(kv.second);
     g_graphs.clear();
+    for (auto& kv : g_prof_graphs) destroy_entry(kv.second);
+    g_prof_graphs.clear();
     return FGPR_OK;
 }
 
@@ -2401,7 +2445,7 @@ int fgpr_set_graph_limit(int n) {
     std::lock_guard<std::mutex> lk(g_graph_mu);
     const int prev = (int)g_graph_limit;
     g_graph_limit = (size_t)n;
-    evict_over_limit(nullptr);
+    evict_over_limit(g_graphs, g_graph_limit, nullptr);
     return prev;
 }
 

@@ -19,13 +19,14 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0
 GRAD_EXACT = 1
 (KERNEL_FWD_H, KERNEL_FWD_E, KERNEL_ADJ_INJECT, KERNEL_ADJ_E, KERNEL_ADJ_H,
  KERNEL_TB_FWD, KERNEL_TB_ADJ) = range(7)
+PROFILE_GRAPH = 0x100      # or'd into a kernel id: replay the timed launches from a graph
 
 
 class Problem(ctypes.Structure):

@@ -3,7 +3,8 @@
 For one grid (NI, NJ, P) and batch size S it times every compiled variant
 (region rows x threads x steps per launch): repeated forward and adjoint
 launches of one block on scratch buffers, split into the shot groups the
-sweep will use and launched back to back, and keeps the fastest per step.
+sweep will use and replayed back to back from captured graphs as the sweeps
+replay theirs, and keeps the fastest per step.
 Results are cached per process and,
 when FASTGPR_TUNE_CACHE names a file, in that JSON file (written atomically).
 """
@@ -53,13 +54,18 @@ def _save_file():
         os.replace(tmp, path)                 # readers never see a partial file
 
 
-def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
+def time_variant(G, er, se, mr, tb, reps: int = 96, graph: bool = True) -> float:
     """Seconds per time step (forward + adjoint launch of one block, divided
     by its steps) of one variant, as the sweep runs it: the batch split into
-    the same shot groups on their streams, each launching its block ``reps``
-    times back to back (fgpr_profile_kernel) after a warm-up; HIP events on
-    the calling stream bracket the fan-out and the join.  Buffers are
-    allocated before the timed region.
+    the same shot groups on their streams, each replaying its block ``reps``
+    times back to back from a captured graph (fgpr_profile_kernel with
+    FGPR_PROFILE_GRAPH), as the sweeps replay theirs; a first, untimed
+    replay captures the graphs.  HIP events on the calling stream bracket
+    the fan-out and the join.  Buffers are allocated before the timed
+    region.  ``graph=False`` issues the launches one by one instead (for
+    comparison: on small grids, where a block takes a few microseconds, it
+    charges each launch a gap the sweeps do not have and picked a variant
+    3-6 % slower at 200 x 100 with 4 and 13 shots).
 
     ``reps`` is sized for a stable ranking: on the 400 x 200, 13-shot
     cross-hole batch, 12 reps put the 6- and 8-step variants of the
@@ -93,18 +99,20 @@ def time_variant(G, er, se, mr, tb, reps: int = 96) -> float:
     for b in groups:
         b.snaps.zero_()
     main = torch.cuda.current_stream(dev)
+    flag = _native.PROFILE_GRAPH if graph else 0
     total = 0.0
     for kid, step in ((_native.KERNEL_TB_FWD, K), (_native.KERNEL_TB_ADJ, 2 * K)):
         def run(b, st, r):
-            _native.check(lib.fgpr_profile_kernel(ctypes.byref(probs[groups.index(b)]), kid, step,
-                                                  r, st), "profile")
+            _native.check(lib.fgpr_profile_kernel(ctypes.byref(probs[groups.index(b)]),
+                                                  kid | flag, step, r, st),
+                          "profile")
 
         def launches(r):
             if n > 1:
                 _solver._fan_out(dev, groups, lambda b, st: run(b, st, r))
             else:
                 run(groups[0], ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), r)
-        launches(2)
+        launches(reps)                        # warm-up; captures each group's graph
         torch.cuda.synchronize(dev)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)

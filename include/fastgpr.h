@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 11
+#define FGPR_ABI_VERSION 12
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -209,11 +209,18 @@ int fgpr_normalise(const double* g, int64_t n, double scale, float* out, void* s
 #define FGPR_KERNEL_TB_FWD 5     /* time-blocked forward block of tb_k steps from step n */
 #define FGPR_KERNEL_TB_ADJ 6     /* time-blocked adjoint block of tb_k steps down from step n */
 
+/* Or'd into fgpr_profile_kernel's `kernel`: replay the launches from a graph. */
+#define FGPR_PROFILE_GRAPH 0x100
+
 /*
  * Launch one solver kernel `reps` times back to back at step n, on the
  * descriptor's buffers, for timing with stream events.  It advances the
- * state like the real step does, so run it on scratch buffers.  No
- * counterpart in the reference (measurement only).
+ * state like the real step does, so run it on scratch buffers.  With
+ * FGPR_PROFILE_GRAPH the `reps` launches are captured once into a HIP graph
+ * (kept in the sweep-graph cache, keyed by the descriptor, kernel, step and
+ * reps) and each call replays it, as the sweeps replay theirs: the first
+ * call with a key also captures, so time the second.  No counterpart in the
+ * reference (measurement only).
  */
 int fgpr_profile_kernel(const fgpr_problem* p, int kernel, int n, int reps, void* stream);
 
@@ -328,10 +335,13 @@ int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, f
  * chunk; the cache holds at most fgpr_set_graph_limit() executables (64 by
  * default) and destroys the least recently launched one beyond that, after
  * its last launch has finished, so callers whose buffers move between calls
- * keep a bounded number of graphs. */
+ * keep a bounded number of graphs.  The graphs of fgpr_profile_kernel
+ * (FGPR_PROFILE_GRAPH) sit in a cache of their own, of at most 8, and are
+ * released here too. */
 int fgpr_release_graphs(void);
 
-/* Number of captured graphs the cache holds now. */
+/* Number of captured sweep graphs the cache holds now (profile graphs not
+ * counted). */
 int fgpr_graph_count(void);
 
 /* Set the cache bound (n >= 1, evicting at once if over it); returns the
