@@ -65,7 +65,7 @@ def time_variant(G, er, se, mr, tb, reps: int = 96, graph: bool = True) -> float
     region.  ``graph=False`` issues the launches one by one instead (for
     comparison: on small grids, where a block takes a few microseconds, it
     charges each launch a gap the sweeps do not have and picked a variant
-    3-6 % slower at 200 x 100 with 4 and 13 shots).
+    6 % slower at 200 x 100 and 800 x 400 with 4 shots).
 
     ``reps`` is sized for a stable ranking: on the 400 x 200, 13-shot
     cross-hole batch, 12 reps put the 6- and 8-step variants of the

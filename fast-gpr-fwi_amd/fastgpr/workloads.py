@@ -142,16 +142,20 @@ def c2_crosshole() -> Workload:
 
 def crosshole(NI: int, NJ: int, S: int, nt: int, nrx: int = 80, P: int = 20) -> Workload:
     """The cross-hole geometry scaled to any grid: S sources down a column at
-    15 % of the width, nrx receivers down a column at 85 %, both over the
-    middle 80 % of the depth; the 8-layer model of seed 1, dx 0.025 m,
-    Ricker 200 MHz.  The autotuner envelope's grids (``bench.py --envelope``)."""
+    15 % of the width, nrx receivers down a column at 85 % (both at least 5
+    cells inside the CPML), over the middle 80 % of the depth (again at
+    least 5 cells inside the CPML); the 8-layer
+    model of seed 1, dx 0.025 m, Ricker 200 MHz.  The autotuner envelope's
+    grids (``bench.py --envelope``)."""
     er, se = layered_blob(NI, NJ, seed=1, P=P)
     dx = 0.025
     dt = 0.9 * float(cfl_dt(dx))
-    si = np.round(np.linspace(0.1 * NI, 0.9 * NI, S)).astype(int)
-    ri = [int(0.1 * NI + r * 0.8 * NI / nrx) for r in range(nrx)]
-    src = _cells([[(int(i), int(0.15 * NJ))] for i in si])
-    rx = _cells([[(i, int(0.85 * NJ)) for i in ri] for _ in si])
+    lo, hi = max(0.1 * NI, P + 5), min(0.9 * NI, NI - P - 6)
+    si = np.round(np.linspace(lo, hi, S)).astype(int)
+    ri = [int(lo + r * (hi - lo) / nrx) for r in range(nrx)]
+    sj, rj = max(int(0.15 * NJ), P + 5), min(int(0.85 * NJ), NJ - P - 6)
+    src = _cells([[(int(i), sj)] for i in si])
+    rx = _cells([[(i, rj) for i in ri] for _ in si])
     return Workload(f"crosshole_{NI}x{NJ}_{S}shots", er.astype(np.float32), se.astype(np.float32),
                     np.ones((NI, NJ), np.float32), _smooth(er, 10).astype(np.float32),
                     _smooth(se, 10).astype(np.float32), dx, dt, nt, P, 2e8, src, rx)

@@ -224,3 +224,24 @@ def test_history_plan_takes_the_largest_ring_that_fits():
             _simulate_history(h)
     with pytest.raises(MemoryError):
         plan(3000, slot, state, 3 * slot)
+
+
+def test_envelope_workloads_are_well_formed():
+    """The autotuner envelope's cross-hole grids (bench.py --envelope): every
+    source and receiver outside the CPML, one source per shot, distinct
+    receivers, the requested step count, and the envelope list itself."""
+    import bench
+    from fastgpr.workloads import crosshole
+    assert len(bench.ENVELOPE) == 15 and (1600, 800, 32, 1500) not in bench.ENVELOPE
+    for NI, NJ, S, nt in bench.ENVELOPE:
+        wl = crosshole(NI, NJ, S, nt)
+        assert (wl.NI, wl.NJ, wl.S, wl.nt, wl.P) == (NI, NJ, S, nt, 20)
+        assert g.n_iterations(wl.time_window, wl.dt) == nt
+        for cells in (wl.src, wl.rx):
+            i, j = cells[..., 0], cells[..., 1]
+            assert (i >= wl.P).all() and (i < NI - wl.P).all()
+            assert (j >= wl.P).all() and (j < NJ - wl.P).all()
+            assert (cells[..., 2] == 0).all()
+        assert wl.src.shape == (S, 1, 3) and wl.rx.shape == (S, 80, 3)
+        assert len({tuple(c) for c in wl.rx[0]}) == 80
+        assert np.isfinite(wl.er).all() and (wl.er >= 1).all() and (wl.se > 0).all()
