@@ -34,6 +34,7 @@ import torch
 
 from . import _native
 from . import solver as _s
+from . import trace as _trace
 from .checkpoint import History, States, plan
 from ._native import check, lib
 
@@ -668,6 +669,10 @@ class Scheduler:
         in one wave.  ``final_ops`` run on the capturing stream after every
         wave has joined.  The launch tables of all ticks are packed on the
         host and uploaded once."""
+        with _trace.span("fgpr uneven capture"):
+            return self._capture(waves, final_ops)
+
+    def _capture(self, waves, final_ops):
         waves = [list(self.jobs)] if waves is None else [list(w) for w in waves]
         everyone = self.jobs
         plans = []
@@ -736,7 +741,8 @@ class CapturedBatch:
         self.ticks, self.launches = ticks, launches
 
     def replay(self):
-        self.graph.replay()
+        with _trace.span("fgpr uneven replay"):
+            self.graph.replay()
 
     def results(self) -> list:
         return [j.results() for j in self.jobs]

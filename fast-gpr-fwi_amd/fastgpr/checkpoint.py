@@ -32,6 +32,7 @@ import torch
 
 from . import _native
 from . import solver as _s
+from . import trace as _trace
 from ._native import check, lib
 
 
@@ -265,11 +266,13 @@ def forward(G, er, se, mr, hist: History):
     ring = _s.device_empty((hist.R,) + b.snap_slot_shape(), torch.float32, dev)
     plain = b.problem(coef, srcscale=scale)
     kept = _ring_problem(b, coef, scale, ring, hist.R)
-    for n0, n1, save, store in forward_cuts(hist):
-        if save:
-            st.save(n0, b)
-        check(lib.fgpr_forward(ctypes.byref(kept if store else plain), n0, n1,
-                               _s._MODES[G.exec_mode], G.graph_chunk, _s._stream(dev)), "forward")
+    with _trace.span(f"fgpr checkpoint forward {_trace.grid_tag(G)} ring {hist.R}"):
+        for n0, n1, save, store in forward_cuts(hist):
+            if save:
+                st.save(n0, b)
+            check(lib.fgpr_forward(ctypes.byref(kept if store else plain), n0, n1,
+                                   _s._MODES[G.exec_mode], G.graph_chunk, _s._stream(dev)),
+                  "forward")
     return b.traces, Forward(b, st, hist, ring, coef, scale)
 
 
@@ -308,8 +311,10 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
     def adj(n_hi, n_lo):
         if ba.tb:
             ba.gshot.zero_()
-        check(lib.fgpr_adjoint(ctypes.byref(pa), n_hi, n_lo, gmode, mode, G.graph_chunk, stream),
-              "adjoint")
+        with _trace.span(f"fgpr {'exact ' if exact else ''}adjoint {_trace.grid_tag(G)} "
+                         f"steps {n_hi}-{n_lo}"):
+            check(lib.fgpr_adjoint(ctypes.byref(pa), n_hi, n_lo, gmode, mode, G.graph_chunk,
+                                   stream), "adjoint")
 
     t = hist.tail()
     if t is not None:
@@ -318,7 +323,9 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
         st.restore(s0, b)
         if s0 >= 1:
             b.seed_slot(pf, (s0 - 1) % R, ring, st.ez(s0))      # Ez after step s0 - 1
-        check(lib.fgpr_forward(ctypes.byref(pf), s0, e, mode, G.graph_chunk, stream), "recompute")
+        with _trace.span(f"fgpr recompute {_trace.grid_tag(G)} steps {s0}-{e}"):
+            check(lib.fgpr_forward(ctypes.byref(pf), s0, e, mode, G.graph_chunk, stream),
+                  "recompute")
         adj(e - 1, max(s0, hist.lowest))
     if exact:
         return _s.finish_exact(G, ge, gs, want_er, want_se)

@@ -52,6 +52,7 @@ import numpy as np
 import torch
 
 from . import batching
+from . import trace as _trace
 from .checkpoint import History
 from .solver import to_device
 from .workloads import Workload
@@ -115,6 +116,16 @@ class Stats:
                 "partial_histories": self.partial_histories,
                 "history_waits": self.history_waits,
                 "peak_admitted_bytes": self.peak_bytes, "max_concurrent_requests": self.max_active}
+
+
+def _host_view(a: np.ndarray) -> torch.Tensor:
+    """A tensor view of a host array that is only read (``to_device`` copies
+    it into pinned memory): a read-only array (a client's shared buffer) is
+    taken without torch's warning about non-writable arrays."""
+    import warnings
+    with warnings.catch_warnings():
+        warnings.filterwarnings("ignore", message="The given NumPy array is not writable")
+        return torch.as_tensor(a)
 
 
 class Server:
@@ -267,7 +278,8 @@ class Server:
                     blocked = True
                 continue
             try:
-                job = self._make_job(req, tb, hist, self.arena.region(off))
+                with _trace.span(f"fgpr admit request {req.rid}"):
+                    job = self._make_job(req, tb, hist, self.arena.region(off))
             except Exception as e:
                 self.arena.release(off)
                 self._arena_version += 1
@@ -304,7 +316,7 @@ class Server:
                                    None, None, None, None, wl.P, ['ricker', 1, wl.freq], wl.S,
                                    er, se, mr, 1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None,
                                    False, False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
-        obs = None if req.obs is None else to_device(torch.as_tensor(req.obs), dev)
+        obs = None if req.obs is None else to_device(_host_view(req.obs), dev)
         job = batching.Job(G, er, se, mr, obs=obs, tb=tb, history=hist, tag=req.rid, alloc=region,
                            keep=lambda shape, dt: torch.empty(shape, dtype=dt, device=dev))
         job.finite_check = True

@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 from . import _native
+from . import trace as _trace
 from ._native import check, lib
 
 _MODES = {"eager": _native.EXEC_EAGER, "graph": _native.EXEC_GRAPH}
@@ -480,6 +481,11 @@ def emitting_sources(G, b: Batch) -> torch.Tensor:
 
 def forward(G, er, se, mr, store: bool = True):
     """Forward sweep over all nt steps.  Returns (traces (S, nt, nrx), batch)."""
+    with _trace.span(f"fgpr forward {_trace.grid_tag(G)}"):
+        return _forward(G, er, se, mr, store)
+
+
+def _forward(G, er, se, mr, store):
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)          # compute.py:729
     coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
@@ -511,6 +517,11 @@ EPS0_KERNEL = 8.8541878128e-12   # the coefficient kernel's float constant (uc.c
 def adjoint_exact(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     """Exact gradient dL/d er, dL/d se of the FP32 forward map (transpose of
     the forward stepping), unnormalised, float32 (NI, NJ, 1)."""
+    with _trace.span(f"fgpr exact adjoint {_trace.grid_tag(G)}"):
+        return _adjoint_exact(G, b, er, se, mr, grad_output, want_er, want_se)
+
+
+def _adjoint_exact(G, b, er, se, mr, grad_output, want_er, want_se):
     dev = G.device
     se_f = torch.clamp(se.detach(), min=0, max=1e6)
     coef = cell_coeffs(er, se_f, mr, G.dt32, G.dx32, False, dev)
@@ -553,6 +564,11 @@ def adjoint(G, b: Batch, er, se, mr, grad_output, want_er=True, want_se=True):
     Returns (grad_er, grad_se) as float32 (NI, NJ, 1) normalised by their L2
     norms, None where not requested, plus the raw FP64 sums.
     """
+    with _trace.span(f"fgpr adjoint {_trace.grid_tag(G)}"):
+        return _adjoint(G, b, er, se, mr, grad_output, want_er, want_se)
+
+
+def _adjoint(G, b, er, se, mr, grad_output, want_er, want_se):
     dev = G.device
     coef = cell_coeffs(er, se, mr, G.dt32, G.dx32, True, dev)      # Ucgeta
     b.reset_state()

@@ -42,6 +42,7 @@ def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
     if not data.is_cuda:
         return total_variation_torch(data, lamda, rho, num_iter, tol)
     from . import _native
+    from . import trace as _trace
     from .solver import _stream
     data.squeeze_(-1)
     x = data.detach().to(torch.float32).contiguous()
@@ -49,10 +50,12 @@ def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
     nbytes = int(_native.lib.fgpr_tv_workspace_bytes(M, N))
     work = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
     out = torch.empty((M, N), dtype=torch.float32, device=x.device)
-    _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
-                                      float(rho), int(num_iter), float(tol),
-                                      ctypes.c_void_p(work.data_ptr()), nbytes,
-                                      ctypes.c_void_p(out.data_ptr()), _stream(x.device)), "tv")
+    with _trace.span(f"fgpr tv {M}x{N}"):
+        _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
+                                          float(rho), int(num_iter), float(tol),
+                                          ctypes.c_void_p(work.data_ptr()), nbytes,
+                                          ctypes.c_void_p(out.data_ptr()), _stream(x.device)),
+                      "tv")
     return out.unsqueeze(-1)
 
 

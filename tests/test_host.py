@@ -245,3 +245,25 @@ def test_envelope_workloads_are_well_formed():
         assert wl.src.shape == (S, 1, 3) and wl.rx.shape == (S, 80, 3)
         assert len({tuple(c) for c in wl.rx[0]}) == 80
         assert np.isfinite(wl.er).all() and (wl.er >= 1).all() and (wl.se > 0).all()
+
+
+def test_trace_spans_nest_and_stay_off_by_default(monkeypatch):
+    """fastgpr.trace: off unless FASTGPR_ROCTX=1; on, the ranges push and pop
+    in pairs (roctx through torch.cuda.nvtx) and an exception still pops."""
+    from fastgpr import trace
+    calls = []
+    monkeypatch.setattr(torch.cuda.nvtx, "range_push", lambda n: calls.append(("push", n)))
+    monkeypatch.setattr(torch.cuda.nvtx, "range_pop", lambda: calls.append(("pop",)))
+    monkeypatch.setattr(trace, "ENABLED", False)
+    with trace.span("off"):
+        pass
+    assert calls == []
+    monkeypatch.setattr(trace, "ENABLED", True)
+    with trace.span("outer"):
+        with trace.span("inner"):
+            pass
+    with pytest.raises(RuntimeError):
+        with trace.span("fails"):
+            raise RuntimeError("x")
+    assert calls == [("push", "outer"), ("push", "inner"), ("pop",), ("pop",),
+                     ("push", "fails"), ("pop",)]
