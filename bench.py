@@ -87,7 +87,8 @@ def parse(argv=None):
     ap.add_argument("--kernel-stats", metavar="DB",
                     help="per (kernel, grid size) dispatch counts and durations of a rocprofv3 "
                          "kernel-trace database; uses no GPU")
-    ap.add_argument("--section", choices=["c3", "one", "c2", "plain", "serve"], default=None,
+    ap.add_argument("--section", choices=["c3", "one", "c2", "plain", "serve", "notebook"],
+                    default=None,
                     help="run one section of the default line and print its JSON (the default "
                          "run starts one process per section)")
     ap.add_argument("--ref-value", type=float, default=None,
@@ -242,6 +243,8 @@ def main():
         out = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline")}
     elif sec == "plain":
         out = plain_path(dev, args.ref_value)
+    elif sec == "notebook":
+        out = notebook_inversion(dev)
     else:
         out = serve_bench(dev, args.serve_check, args.serve_rate, args.serve_histories)
     print(json.dumps(out), flush=True)
@@ -251,8 +254,9 @@ def main():
 # order): the headline, then the extra keys.  A section that fails leaves an
 # "error" in its key and the others still run.
 SECTIONS = (("c3", None), ("one", "one_model_at_a_time"), ("c2", "c2_crosshole"),
-            ("plain", "plain_path"), ("serve", "serving"))
-SECTION_TIMEOUT_S = {"c3": 420, "one": 300, "c2": 240, "plain": 240, "serve": 300}
+            ("plain", "plain_path"), ("serve", "serving"), ("notebook", "notebook_inversion"))
+SECTION_TIMEOUT_S = {"c3": 420, "one": 300, "c2": 240, "plain": 240, "serve": 300,
+                     "notebook": 240}
 
 
 def run_sections(args) -> int:
@@ -869,6 +873,76 @@ def tuner_envelope(dev, cases=ENVELOPE, grad_reps=3):
 def _native_tiles(G, tb):
     from fastgpr import _native
     _native.tb_tiles(G.NI, G.NJ, *tb)
+
+
+# The reference's only recorded timings: its example notebook's epoch wall
+# times (unstated NVIDIA GPU; example.ipynb:145, 148 for phase 1, :295-748 for
+# phase 2) and 262.99 s for all 202 epochs (:748).
+NOTEBOOK_REFERENCE = {"phase1_epoch_median_s": 1.15, "phase2_epoch_median_s": 1.33,
+                      "total_202_epochs_s": 262.99, "hardware": "unstated NVIDIA GPU",
+                      "source": "example.ipynb:145-748 (recorded stdout)"}
+
+
+def notebook_inversion(dev, phase1=50, phase2=152):
+    """The reference notebook's inversion (example.ipynb cells 0-1) through
+    ``fastgpr.inversion``: 220 x 120 padded models rebuilt from the
+    reference's newmodel.npy (tests/golden/notebook_models.npz), 40 shots,
+    200 fixed receivers, 10-cell CPML, dt 1e-10 s, 1001 steps, TV on, Adam.
+    ``phase1`` epochs on epsilon, then ``phase2`` on epsilon and sigma (the
+    notebook's 50 + 152); median epoch wall time per phase and the total
+    beside the notebook's recorded ones, and the loss of every epoch against
+    the loss the notebook printed for it."""
+    import numpy as np
+    import torch
+    from fastgpr.compute import compute
+    from fastgpr.inversion import InversionConfig, invert
+    d = np.load(Path(__file__).resolve().parent / "tests" / "golden" / "notebook_models.npz",
+                allow_pickle=False)
+    t = lambda k: torch.tensor(d[k], device=dev).unsqueeze(-1)          # float64, as the notebook
+    ep_t, se_t, ep_init, se_init = t("ep_true"), t("se_true"), t("ep_init"), t("se_init")
+    mr = torch.ones_like(ep_t)
+    dx, nx, ny, nz = torch.tensor(0.05), torch.tensor(220), torch.tensor(120), torch.tensor(1)
+    kw = dict(nsrc=1, nrx=200, time_windows=torch.tensor(1e-7), dt=1e-10,
+              source_location=torch.tensor([0.5, 0.50, 0]), source_step=torch.tensor([0.25, 0, 0]),
+              receiver_location=torch.tensor([0.5, 5.50, 0]),
+              receiver_step=torch.tensor([0.05, 0, 0]), pmlthick=10, Fixedreceiver=True,
+              time_block="auto")
+    args = (dev, dx, ny, nx, nz, ['hertzian', 'z'], ['ricker', 1, 1e8], 40)
+    with torch.no_grad():
+        observed, _ = compute(*args, ep_t, se_t, mr, 1, 200,
+                              **{k: v for k, v in kw.items() if k not in ("nsrc", "nrx")})
+    torch.cuda.synchronize()
+    cfg = InversionConfig(n_epochs=phase1 + phase2, phase1_epochs=phase1, pmlthick=10)
+    out = invert(args, kw, ep_init, se_init, mr, observed, cfg)
+    sec = out["epoch_seconds"]
+    p1, p2 = float(np.median(sec[:phase1])), float(np.median(sec[phase1:]))
+    cu = 2 * 40 * 220 * 120 * 1001            # cell updates of one gradient (forward + adjoint)
+    ref = NOTEBOOK_REFERENCE
+    got = np.asarray(out["losses"], np.float64)
+    rec = np.asarray(d["losses"][:len(got)], np.float64)
+    rel = np.abs(got - rec) / rec
+    res = {"what": "the reference notebook's inversion (forward + adjoint + TV + Adam per epoch, "
+                   "epsilon only in phase 1, epsilon and sigma in phase 2), wall time per epoch",
+           "epochs": [phase1, phase2], "first_epoch_s": round(sec[0], 4),
+           "phase1_epoch_median_s": round(p1, 4), "phase2_epoch_median_s": round(p2, 4),
+           "total_s": round(float(np.sum(sec)), 3),
+           "phase1_gcell_updates_per_s": round(cu / p1 / 1e9, 1),
+           "reference": ref,
+           "speedup_vs_notebook_epochs": [round(ref["phase1_epoch_median_s"] / p1, 1),
+                                          round(ref["phase2_epoch_median_s"] / p2, 1)],
+           "loss_vs_notebook": {
+               "epochs_compared": int(len(got)),
+               "rel_diff": {str(e): float(f"{rel[e]:.2e}") for e in (0, 1, 2, 10, 49, 50, 100, 201)
+                            if e < len(got)},
+               "max_rel_diff_phase1": float(f"{rel[:phase1].max():.2e}"),
+               "first_epoch_over_1e-3": (int(np.argmax(rel > 1e-3)) if (rel > 1e-3).any()
+                                         else None),
+               "last": [round(float(got[-1]), 6), round(float(rec[-1]), 6)],
+               "mean_last_50": [round(float(got[-50:].mean()), 4),
+                                round(float(rec[-50:].mean()), 4)]}}
+    if phase1 + phase2 == 202:
+        res["speedup_vs_notebook_total"] = round(ref["total_202_epochs_s"] / float(np.sum(sec)), 1)
+    return res
 
 
 def plain_path(dev, value):
