@@ -1615,6 +1615,9 @@ std::map<GraphKey, GraphEntry> g_graphs;
 // timing (the autotuner) never evicts a sweep's graphs.
 std::map<GraphKey, GraphEntry> g_prof_graphs;
 constexpr size_t kProfGraphLimit = 8;
+// Graphs of fgpr_tv's launch sequence (about 2 per ADMM iteration).
+std::map<GraphKey, GraphEntry> g_tv_graphs;
+constexpr size_t kTVGraphLimit = 8;
 uint64_t g_graph_clock = 0;
 size_t g_graph_limit = 64;
 
@@ -2305,8 +2308,13 @@ int64_t fgpr_tv_workspace_bytes(int M, int N) {
     return tv_bytes(M, N);
 }
 
-int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
-            void* workspace, int64_t bytes, float* out, void* stream) {
+// The TV launches of one call.  `cache`: replay them from a captured graph
+// (kept in a cache of its own, keyed by the pointers, sizes and parameters)
+// unless the stream is being captured by the caller; otherwise launch them
+// one by one (the allocation-free entry point: instantiating a graph
+// allocates).
+static int tv_run(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+                  void* workspace, int64_t bytes, float* out, hipStream_t st, bool cache) {
     if (!data || !out || !workspace) return fail(FGPR_EINVAL, "null pointer");
     if (M < 1 || N < 1 || num_iter < 1 || !(rho > 0.f)) return fail(FGPR_EINVAL, "bad TV arguments");
     if (bytes < tv_bytes(M, N)) return fail(FGPR_EINVAL, "TV workspace too small");
@@ -2320,24 +2328,43 @@ int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_ite
     w.scal = (float*)(base + 7 * pl);
     w.flags = (int*)(base + 7 * pl + 64);
     w.part = (double*)(base + 7 * pl + 256);
-    hipStream_t st = (hipStream_t)stream;
     const dim3 blk(kTVBx, kTVBy), grid((w.N2 + kTVBx - 1) / kTVBx, (w.M2 + kTVBy - 1) / kTVBy);
     // float32 constants as the reference forms them: lamda * rho and
     // 1 + 4 * lamda * rho in double, then rounded; 1 / rho likewise
     const float lr = (float)((double)lamda * (double)rho);
     const float den = (float)(1.0 + 4.0 * (double)lamda * (double)rho);
     const float inv_rho = (float)(1.0 / (double)rho);
-    hipLaunchKernelGGL(k_tv_max, dim3(1), dim3(1024), 0, st, data, (int64_t)M * N, w);
-    hipLaunchKernelGGL(k_tv_init, grid, blk, 0, st, data, w);
     const int nparts = (int)tv_blocks(M, N);
-    for (int k = 0; k < num_iter; ++k) {
-        hipLaunchKernelGGL(k_tv_x, grid, blk, 0, st, w, k, lamda, lr, den, rho);
-        hipLaunchKernelGGL(k_tv_zu, grid, blk, 0, st, w, k, rho, inv_rho, nparts, (double)tol);
+    auto body = [&](hipStream_t s) {
+        hipLaunchKernelGGL(k_tv_max, dim3(1), dim3(1024), 0, s, data, (int64_t)M * N, w);
+        hipLaunchKernelGGL(k_tv_init, grid, blk, 0, s, data, w);
+        for (int k = 0; k < num_iter; ++k) {
+            hipLaunchKernelGGL(k_tv_x, grid, blk, 0, s, w, k, lamda, lr, den, rho);
+            hipLaunchKernelGGL(k_tv_zu, grid, blk, 0, s, w, k, rho, inv_rho, nparts, (double)tol);
+        }
+        hipLaunchKernelGGL(k_tv_out, dim3((N + kTVBx - 1) / kTVBx, (M + kTVBy - 1) / kTVBy), blk, 0,
+                           s, w, num_iter, out);
+        HIP_TRY(hipGetLastError());
+        return FGPR_OK;
+    };
+    if (cache) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(st, &cap));
+        cache = cap == hipStreamCaptureStatusNone;
     }
-    hipLaunchKernelGGL(k_tv_out, dim3((N + kTVBx - 1) / kTVBx, (M + kTVBy - 1) / kTVBy), blk, 0,
-                       st, w, num_iter, out);
-    HIP_TRY(hipGetLastError());
-    return FGPR_OK;
+    if (!cache) return body(st);
+    GraphKey key;
+    auto I = [&](int64_t x) { key.v.push_back(x); };
+    auto F = [&](float x) { int32_t b; std::memcpy(&b, &x, 4); key.v.push_back(b); };
+    I((int64_t)(uintptr_t)data); I((int64_t)(uintptr_t)workspace); I((int64_t)(uintptr_t)out);
+    I(M); I(N); F(lamda); F(rho); I(num_iter); F(tol);
+    return launch_cached(g_tv_graphs, kTVGraphLimit, std::move(key), body, 1, st);
+}
+
+int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
+            void* workspace, int64_t bytes, float* out, void* stream) {
+    return tv_run(data, M, N, lamda, rho, num_iter, tol, workspace, bytes, out, (hipStream_t)stream,
+                  true);
 }
 
 int64_t fgpr_tb_map(const fgpr_problem* p, int adj, int32_t* shot, int32_t* tile, int8_t* interior,
@@ -2382,8 +2409,8 @@ int fgpr_tv_static(const float* data, int M, int N, float lamda, float rho, int 
     // every word of the workspace the kernels read is written first in the
     // same call (k_tv_max: the maximum and the flags; k_tv_init: the seven
     // planes; each X sweep: its block sums), so nothing is cleared here
-    return fgpr_tv(data, M, N, lamda, rho, num_iter, tol, ws, (int64_t)FGPR_TV_STATIC_BYTES, out,
-                   stream);
+    return tv_run(data, M, N, lamda, rho, num_iter, tol, ws, (int64_t)FGPR_TV_STATIC_BYTES, out,
+                  (hipStream_t)stream, false);
 }
 
 int64_t fgpr_snap_slot_floats(const fgpr_problem* p) {
@@ -2432,6 +2459,8 @@ int fgpr_release_graphs(void) {
     g_graphs.clear();
     for (auto& kv : g_prof_graphs) destroy_entry(kv.second);
     g_prof_graphs.clear();
+    for (auto& kv : g_tv_graphs) destroy_entry(kv.second);
+    g_tv_graphs.clear();
     return FGPR_OK;
 }
 

@@ -45,18 +45,40 @@ def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
     from . import trace as _trace
     from .solver import _stream
     data.squeeze_(-1)
-    x = data.detach().to(torch.float32).contiguous()
-    M, N = x.shape
-    nbytes = int(_native.lib.fgpr_tv_workspace_bytes(M, N))
-    work = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
-    out = torch.empty((M, N), dtype=torch.float32, device=x.device)
+    M, N = data.shape
+    x, work, out = _buffers(data.device, M, N)
+    x.copy_(data.detach())                    # float32, rounded as .to(torch.float32) rounds
+    nbytes = work.numel()
     with _trace.span(f"fgpr tv {M}x{N}"):
         _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
                                           float(rho), int(num_iter), float(tol),
                                           ctypes.c_void_p(work.data_ptr()), nbytes,
                                           ctypes.c_void_p(out.data_ptr()), _stream(x.device)),
                       "tv")
-    return out.unsqueeze(-1)
+    return out.clone().unsqueeze(-1)
+
+
+# Input, workspace and output buffers kept per (device, stream, grid): the
+# same pointers on every call, so the library replays the call's captured
+# graph instead of capturing a new one.  Calls on one stream are ordered, so
+# a call may reuse the buffers of the one before; the result is a copy.
+_BUFFERS: dict = {}
+_BUFFER_SETS = 8
+
+
+def _buffers(device, M: int, N: int):
+    from . import _native
+    key = (str(device), torch.cuda.current_stream(device).cuda_stream, M, N)
+    b = _BUFFERS.get(key)
+    if b is None:
+        nbytes = int(_native.lib.fgpr_tv_workspace_bytes(M, N))
+        b = (torch.empty((M, N), dtype=torch.float32, device=device),
+             torch.empty(nbytes, dtype=torch.uint8, device=device),
+             torch.empty((M, N), dtype=torch.float32, device=device))
+        if len(_BUFFERS) >= _BUFFER_SETS:
+            _BUFFERS.pop(next(iter(_BUFFERS)))
+        _BUFFERS[key] = b
+    return b
 
 
 def total_variation_torch(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,

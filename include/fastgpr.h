@@ -264,7 +264,12 @@ int fgpr_reduce_shots(const fgpr_problem* p, void* stream);
  * when ||X - X_prev||_2 < tol; out = X * max.  data and out are M*N floats
  * (row-major, the model without its trailing unit axis).  The iterations run
  * on the device without host synchronisation; workspace is caller-owned
- * device memory of fgpr_tv_workspace_bytes(M, N) bytes, 256-byte aligned. */
+ * device memory of fgpr_tv_workspace_bytes(M, N) bytes, 256-byte aligned.
+ * The call's launches (about two per iteration) are captured once into a
+ * HIP graph, cached by (data, workspace, out, M, N, lamda, rho, num_iter,
+ * tol) in a cache of at most 8, and replayed; a caller that keeps those
+ * buffers in place replays one graph per call.  On a stream the caller is
+ * capturing, the launches go into the caller's capture instead. */
 int64_t fgpr_tv_workspace_bytes(int M, int N);
 int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
             void* workspace, int64_t bytes, float* out, void* stream);
@@ -278,7 +283,8 @@ int fgpr_tv(const float* data, int M, int N, float lamda, float rho, int num_ite
  * 2.39 million padded cells, e.g. 2000 x 1000) are rejected with
  * FGPR_EINVAL.  Calls on one device share the array: calls that overlap in
  * time on one device (two streams, two threads) race and must be ordered by
- * the caller.  Same results as fgpr_tv, bitwise. */
+ * the caller.  Same results as fgpr_tv, bitwise.  It launches its kernels
+ * one by one (no graph is captured or instantiated). */
 #define FGPR_TV_STATIC_BYTES (64u << 20)
 int fgpr_tv_static(const float* data, int M, int N, float lamda, float rho, int num_iter, float tol,
                    float* out, void* stream);
@@ -336,8 +342,8 @@ int fgpr_fir(const float* x, int S, int nt, int nrx, const float* taps, int L, f
  * default) and destroys the least recently launched one beyond that, after
  * its last launch has finished, so callers whose buffers move between calls
  * keep a bounded number of graphs.  The graphs of fgpr_profile_kernel
- * (FGPR_PROFILE_GRAPH) sit in a cache of their own, of at most 8, and are
- * released here too. */
+ * (FGPR_PROFILE_GRAPH) and of fgpr_tv sit in caches of their own, of at most
+ * 8 each, and are released here too. */
 int fgpr_release_graphs(void);
 
 /* Number of captured sweep graphs the cache holds now (profile graphs not
