@@ -87,7 +87,7 @@ def parse(argv=None):
     ap.add_argument("--kernel-stats", metavar="DB",
                     help="per (kernel, grid size) dispatch counts and durations of a rocprofv3 "
                          "kernel-trace database; uses no GPU")
-    ap.add_argument("--section", choices=["c3", "one", "c2", "plain", "serve", "notebook"],
+    ap.add_argument("--section", choices=["c3", "one", "c2", "plain", "serve", "notebook", "c4"],
                     default=None,
                     help="run one section of the default line and print its JSON (the default "
                          "run starts one process per section)")
@@ -245,6 +245,8 @@ def main():
         out = plain_path(dev, args.ref_value)
     elif sec == "notebook":
         out = notebook_inversion(dev)
+    elif sec == "c4":
+        out = c4_memory_bound(dev)
     else:
         out = serve_bench(dev, args.serve_check, args.serve_rate, args.serve_histories)
     print(json.dumps(out), flush=True)
@@ -254,9 +256,10 @@ def main():
 # order): the headline, then the extra keys.  A section that fails leaves an
 # "error" in its key and the others still run.
 SECTIONS = (("c3", None), ("one", "one_model_at_a_time"), ("c2", "c2_crosshole"),
-            ("plain", "plain_path"), ("serve", "serving"), ("notebook", "notebook_inversion"))
+            ("plain", "plain_path"), ("serve", "serving"), ("notebook", "notebook_inversion"),
+            ("c4", "c4_memory_bound"))
 SECTION_TIMEOUT_S = {"c3": 420, "one": 300, "c2": 240, "plain": 240, "serve": 300,
-                     "notebook": 240}
+                     "notebook": 240, "c4": 300}
 
 
 def run_sections(args) -> int:
@@ -873,6 +876,60 @@ def tuner_envelope(dev, cases=ENVELOPE, grad_reps=3):
 def _native_tiles(G, tb):
     from fastgpr import _native
     _native.tb_tiles(G.NI, G.NJ, *tb)
+
+
+def c4_memory_bound(dev, reps=2):
+    """The 2000 x 1000, 4-shot, 8000-step gradient three ways: the
+    reference gradient with its whole snapshot history in HBM (256 GB:
+    ``checkpoint="auto"`` keeps it on a 288 GB device), the reference
+    gradient with a checkpoint every 90 steps (recompute), and the exact
+    gradient (the finite-difference-checked mode) with a checkpoint every 96
+    steps.  Seconds per gradient after one warm-up, best of ``reps``."""
+    import torch
+    import fastgpr
+    from fastgpr import checkpoint as ck
+    from fastgpr.workloads import c4_memory
+    wl = c4_memory()
+    er_t, se_t, mr = _tensors(wl, dev, (wl.er, wl.se))
+    er0, se0, _ = _tensors(wl, dev, (wl.er0, wl.se0))
+    with torch.no_grad():
+        obs = _compute(wl, dev, er_t, se_t, mr, time_block="auto").detach().clone()
+    del er_t, se_t
+    cu = wl.cell_updates_per_gradient()
+    out = {"workload": wl.name, "grid": [wl.NI, wl.NJ], "shots": wl.S, "steps": wl.nt,
+           "cell_updates_per_gradient": cu}
+    modes = (("reference_whole_history", dict(checkpoint="auto")),
+             ("reference_checkpoint_90", dict(checkpoint=90)),
+             ("exact_checkpoint_96", dict(gradient="exact", checkpoint=96, graph_chunk=48)))
+    for name, kw in modes:
+        er = er0.clone().requires_grad_(True)
+        se = se0.clone().requires_grad_(True)
+        times = []
+        for r in range(reps + 1):
+            er.grad = None
+            se.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pred = _compute(wl, dev, er, se, mr, time_block="auto", **kw)
+            (0.5 * ((pred - obs) ** 2).sum()).backward()
+            torch.cuda.synchronize()
+            if r:
+                times.append(time.perf_counter() - t0)
+        assert torch.isfinite(er.grad).all() and torch.isfinite(se.grad).all()
+        best = min(times)
+        rec = {"seconds": round(best, 4), "gcell_updates_per_s": round(cu / best / 1e9, 1)}
+        if kw.get("checkpoint") == "auto":
+            G = fastgpr.initialization(dev, wl.dx, wl.NJ, wl.NI, 1, wl.time_window,
+                                       ['hertzian', 'z'], None, None, None, None, wl.P,
+                                       ['ricker', 1, wl.freq], wl.S, er.detach(), se.detach(), mr,
+                                       1, wl.rx.shape[1], wl.dt, 'HORIPML', None, None, False,
+                                       False, None, torch.tensor(wl.src), torch.tensor(wl.rx))
+            rec["whole_history_kept"] = ck.history(G, "auto") is None
+            rec["history_gb"] = round(wl.S * wl.nt * wl.NI * wl.NJ * 4 / 1e9, 1)
+        out[name] = rec
+        del er, se, pred
+        torch.cuda.empty_cache()
+    return out
 
 
 # The reference's only recorded timings: its example notebook's epoch wall

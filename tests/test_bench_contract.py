@@ -136,7 +136,7 @@ def test_default_line_merges_sections_and_survives_a_failed_one(monkeypatch, cap
                                           "batched_speedup": 2.0}
     assert "error" in out["c2_crosshole"]
     assert out["plain_path"] == {"value": 1.0} and out["serving"] == {"value": 1.0}
-    assert out["notebook_inversion"] == {"value": 1.0}
+    assert out["notebook_inversion"] == {"value": 1.0} and out["c4_memory_bound"] == {"value": 1.0}
     plain = dict(seen)["plain"]
     assert plain[plain.index("--ref-value") + 1] == "200.0"
-    assert [s for s, _ in seen] == ["c3", "one", "c2", "plain", "serve", "notebook"]
+    assert [s for s, _ in seen] == ["c3", "one", "c2", "plain", "serve", "notebook", "c4"]
