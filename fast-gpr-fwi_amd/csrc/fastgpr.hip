@@ -423,6 +423,17 @@ struct XScratch {
     float* U; float* Lx; float* Ly; float* Vx; float* Vy; float* Mx; float* My;
 };
 
+// Whether E row / column q (0 <= q < n) lies in a CPML slab: [1, P] or
+// [n - P, n) (the E slab indices of k_xadj_a, per axis).
+__device__ __forceinline__ bool xslab_e(int q, int n, int P) {
+    return P > 0 && ((q >= 1 && q <= P) || q >= n - P);
+}
+
+// The same for H rows / columns: [0, P) or [n - P, n).
+__device__ __forceinline__ bool xslab_h(int q, int n, int P) {
+    return P > 0 && (q < P || q >= n - P);
+}
+
 __device__ __forceinline__ XScratch xscratch(const Dev& d, float* base) {
     const size_t f = (size_t)d.S * plane(d);
     XScratch x;
@@ -441,7 +452,6 @@ k_xadj_a(Dev d, int koff, int dir, float* scratch) {
     const XScratch x = xscratch(d, scratch);
     const size_t pl = plane(d);
     const size_t o = (size_t)i * pitch + j;
-    const float uE1 = d.coef[pl + o];
     const float uE4 = d.coef[2 * pl + o];
     const bool mainE = (i >= 1 && j >= 1);
     int xq = -1, xk = 0, yq = -1, yk = 0;
@@ -460,22 +470,20 @@ k_xadj_a(Dev d, int koff, int dir, float* scratch) {
         const double ue = (double)uE4 * (double)e;
         acc_e += ue * (f1 - f0);
         acc_s += mainE ? -0.5 * ue * (f1 + f0) : -0.5 * ue * (f1 - f0);
-        x.U[b] = mainE ? uE1 * e : 0.f;
-        float lx = 0.f, ly = 0.f;
+        // U = uE1 * lambda_E is formed where it is used (k_xadj_b); Lx / Ly
+        // exist in the CPML slabs only and are read only there
         if (xq >= 0) {
             float* pm = psi_x(d, xk, s, xq, j);
             const float m = *pm;
-            lx = uE4 * (d.pml[xq] - 1.0f) * e - d.pml[3 * P + xq] * m;
+            x.Lx[b] = uE4 * (d.pml[xq] - 1.0f) * e - d.pml[3 * P + xq] * m;
             *pm = uE4 * d.pml[P + xq] * e + d.pml[2 * P + xq] * m;
         }
         if (yq >= 0) {
             float* pm = psi_y(d, yk, s, i, yq);
             const float m = *pm;
-            ly = -uE4 * (d.pml[yq] - 1.0f) * e - d.pml[3 * P + yq] * m;
+            x.Ly[b] = -uE4 * (d.pml[yq] - 1.0f) * e - d.pml[3 * P + yq] * m;
             *pm = -uE4 * d.pml[P + yq] * e + d.pml[2 * P + yq] * m;
         }
-        x.Lx[b] = lx;
-        x.Ly[b] = ly;
     }
     const size_t g = (size_t)i * NJ + j;
     d.grad_eps[g] += acc_e;
@@ -491,7 +499,6 @@ k_xadj_b(Dev d, float* scratch) {
     const XScratch x = xscratch(d, scratch);
     const size_t pl = plane(d);
     const size_t o = (size_t)i * pitch + j;
-    const float uH1 = d.coef[3 * pl + o];
     const float uH4 = d.coef[4 * pl + o];
     const float idx = d.idx;
     int xq = -1, xk = 0, yq = -1, yk = 0;
@@ -501,10 +508,25 @@ k_xadj_b(Dev d, float* scratch) {
         if (j < P) { yq = P - 1 - j; yk = HY0; }
         else if (j >= NJ - P) { yq = j - (NJ - P); yk = HYM; }
     }
+    // U = uE1 * lambda_E of the E cells (i, j), (i + 1, j), (i, j + 1): zero
+    // off the main E update (row 0, column 0) and outside the grid; Lx / Ly
+    // of the same cells: zero outside the E slabs (k_xadj_a writes them
+    // there only)
+    const float e1c = d.coef[pl + o];
+    const float e1d = d.coef[pl + o + pitch];
+    const float e1r = d.coef[pl + o + 1];
+    const bool uc = i >= 1 && j >= 1, ud = i + 1 < NI && j >= 1, ur = j + 1 < NJ && i >= 1;
+    const bool lxc = xslab_e(i, NI, P), lxd = i + 1 < NI && xslab_e(i + 1, NI, P);
+    const bool lyc = xslab_e(j, NJ, P), lyr = j + 1 < NJ && xslab_e(j + 1, NJ, P);
     for (int s = 0; s < d.S; ++s) {
         const size_t b = (size_t)s * pl + o;
-        float hy = d.Hy[b] + (x.U[b] - x.U[b + pitch]) + (x.Lx[b] - x.Lx[b + pitch]) * idx;
-        float hx = d.Hx[b] + (x.U[b + 1] - x.U[b]) + (x.Ly[b] - x.Ly[b + 1]) * idx;
+        const float u0 = uc ? e1c * d.Ez[b] : 0.f;
+        const float u_d = ud ? e1d * d.Ez[b + pitch] : 0.f;
+        const float u_r = ur ? e1r * d.Ez[b + 1] : 0.f;
+        const float lx0 = lxc ? x.Lx[b] : 0.f, lx_d = lxd ? x.Lx[b + pitch] : 0.f;
+        const float ly0 = lyc ? x.Ly[b] : 0.f, ly_r = lyr ? x.Ly[b + 1] : 0.f;
+        float hy = d.Hy[b] + (u0 - u_d) + (lx0 - lx_d) * idx;
+        float hx = d.Hx[b] + (u_r - u0) + (ly0 - ly_r) * idx;
         // Hx on row 0 and Hy on column 0 belong to the closed, identically
         // zero boundary subsystems of the forward map (only CPML terms update
         // them); their adjoints never reach the interior, have a zero
@@ -514,23 +536,20 @@ k_xadj_b(Dev d, float* scratch) {
         if (j == 0) hy = 0.f;
         d.Hy[b] = hy;
         d.Hx[b] = hx;
-        float mx = 0.f, my = 0.f;
+        // Mx / My in the H slabs only (read only there); V = uH1 * lambda_H is
+        // formed where it is used (k_xadj_c)
         if (xq >= 0) {
             float* pm = psi_x(d, xk, s, xq, j);
             const float m = *pm;
-            mx = uH4 * (d.pml[4 * P + xq] - 1.0f) * hy - d.pml[7 * P + xq] * m;
+            x.Mx[b] = uH4 * (d.pml[4 * P + xq] - 1.0f) * hy - d.pml[7 * P + xq] * m;
             *pm = uH4 * d.pml[5 * P + xq] * hy + d.pml[6 * P + xq] * m;
         }
         if (yq >= 0) {
             float* pm = psi_y(d, yk, s, i, yq);
             const float m = *pm;
-            my = -uH4 * (d.pml[4 * P + yq] - 1.0f) * hx - d.pml[7 * P + yq] * m;
+            x.My[b] = -uH4 * (d.pml[4 * P + yq] - 1.0f) * hx - d.pml[7 * P + yq] * m;
             *pm = -uH4 * d.pml[5 * P + yq] * hx + d.pml[6 * P + yq] * m;
         }
-        x.Mx[b] = mx;
-        x.My[b] = my;
-        x.Vx[b] = i >= 1 ? uH1 * hx : 0.f;
-        x.Vy[b] = j >= 1 ? uH1 * hy : 0.f;
     }
 }
 
@@ -545,11 +564,31 @@ k_xadj_c(Dev d, float* scratch) {
     const size_t o = (size_t)i * pitch + j;
     const float c0 = (i >= 1 && j >= 1) ? d.coef[o] : 1.0f;
     const float idx = d.idx;
+    const int P = d.P;
+    // Vx = uH1 * lambda_Hx (zero on row 0), Vy = uH1 * lambda_Hy (zero on
+    // column 0) of the H cells (i, j), (i, j - 1), (i - 1, j); Mx / My: zero
+    // outside the H slabs
+    const float h1c = d.coef[3 * pl + o];
+    const float h1l = j >= 1 ? d.coef[3 * pl + o - 1] : 0.f;
+    const float h1u = i >= 1 ? d.coef[3 * pl + o - pitch] : 0.f;
+    const bool mxc = xslab_h(i, NI, P), mxu = i >= 1 && xslab_h(i - 1, NI, P);
+    const bool myc = xslab_h(j, NJ, P), myl = j >= 1 && xslab_h(j - 1, NJ, P);
     for (int s = 0; s < d.S; ++s) {
         const size_t b = (size_t)s * pl + o;
-        float v = c0 * d.Ez[b] + x.Vx[b] - x.Vy[b] - x.Mx[b] * idx - x.My[b] * idx;
-        if (j >= 1) v = v - x.Vx[b - 1] + x.My[b - 1] * idx;
-        if (i >= 1) v = v + x.Vy[b - pitch] + x.Mx[b - pitch] * idx;
+        const float vx = i >= 1 ? h1c * d.Hx[b] : 0.f;
+        const float vy = j >= 1 ? h1c * d.Hy[b] : 0.f;
+        const float mx = mxc ? x.Mx[b] : 0.f, my = myc ? x.My[b] : 0.f;
+        float v = c0 * d.Ez[b] + vx - vy - mx * idx - my * idx;
+        if (j >= 1) {
+            const float vx_l = i >= 1 ? h1l * d.Hx[b - 1] : 0.f;
+            const float my_l = myl ? x.My[b - 1] : 0.f;
+            v = v - vx_l + my_l * idx;
+        }
+        if (i >= 1) {
+            const float vy_u = j >= 1 ? h1u * d.Hy[b - pitch] : 0.f;
+            const float mx_u = mxu ? x.Mx[b - pitch] : 0.f;
+            v = v + vy_u + mx_u * idx;
+        }
         d.Ez[b] = (i >= 1 && j >= 1) ? v : 0.f;   // boundary subsystem, see k_xadj_b
     }
 }
