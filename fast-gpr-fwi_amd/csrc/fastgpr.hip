@@ -408,19 +408,23 @@ __global__ void k_adj_inject(Dev d, int koff, int dir) {
 // gradients that finite differences of the forward map can check.  Reverse
 // step n takes the adjoints of (E^{n+1}, H^{n+1/2}, psi) to those of
 // (E^n, H^{n-1/2}, psi) in four launches:
-//   A: per E cell -- gradient terms, the curl coefficient times lambda (U),
-//      the E-CPML transposes (Lx, Ly) and the psi_E adjoints;
-//   B: per H cell -- gather U / Lx / Ly into lambda_H, H-CPML transposes
-//      (Mx, My, psi_H adjoints) and the H curl terms (Vx, Vy);
-//   C: per E cell -- lambda_E = (uE0 or 1) lambda + gathers of V / M;
+//   A: per E cell -- gradient terms, the E-CPML transposes (Lx, Ly, in the
+//      slabs) and the psi_E adjoints;
+//   B: per H cell -- gather U = uE1 * lambda_E (formed from lambda_E) and
+//      Lx / Ly into lambda_H, then the H-CPML transposes (Mx, My, in the
+//      slabs) and the psi_H adjoints;
+//   C: per E cell -- lambda_E = (uE0 or 1) lambda + gathers of
+//      V = uH1 * lambda_H (formed from lambda_H) and M;
 //   D: lambda_E at the receivers += dL/dtraces[n].
+// The scratch holds the four slab planes Lx, Ly, Mx, My; every entry read is
+// written earlier in the same step, so it needs no initialisation.
 // dE^{n+1}/d er = -(e0/dt) uE4 (E^{n+1} - E^n) and, on cells the main update
 // covers, dE^{n+1}/d se = -uE4 (E^{n+1} + E^n)/2 (every other term of the E
 // update is uE4 or uE1 = uE4/dx times something independent of the cell's
 // own er / se).  grad_eps accumulates sum uE4*lambda*(E^{n+1}-E^n); the
 // constant -(e0/dt) is applied by the caller.
 struct XScratch {
-    float* U; float* Lx; float* Ly; float* Vx; float* Vy; float* Mx; float* My;
+    float* Lx; float* Ly; float* Mx; float* My;
 };
 
 // Whether E row / column q (0 <= q < n) lies in a CPML slab: [1, P] or
@@ -437,8 +441,7 @@ __device__ __forceinline__ bool xslab_h(int q, int n, int P) {
 __device__ __forceinline__ XScratch xscratch(const Dev& d, float* base) {
     const size_t f = (size_t)d.S * plane(d);
     XScratch x;
-    x.U = base; x.Lx = base + f; x.Ly = base + 2 * f; x.Vx = base + 3 * f;
-    x.Vy = base + 4 * f; x.Mx = base + 5 * f; x.My = base + 6 * f;
+    x.Lx = base; x.Ly = base + f; x.Mx = base + 2 * f; x.My = base + 3 * f;
     return x;
 }
 

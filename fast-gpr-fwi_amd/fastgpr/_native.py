@@ -19,7 +19,7 @@ import torch  # noqa: F401
 
 _LIB_PATH = Path(os.environ.get("FASTGPR_LIB", Path(__file__).with_name("libfastgpr.so")))
 
-ABI_VERSION = 12
+ABI_VERSION = 13
 EXEC_EAGER = 0
 EXEC_GRAPH = 1
 GRAD_REFERENCE = 0

@@ -303,7 +303,7 @@ def adjoint(G, fw: Forward, er, se, mr, grad_output, want_er=True, want_se=True,
     pa.snaps, pa.nsnap, pa.snap_mod = ring.data_ptr(), R, R
     scratch = None
     if exact:
-        scratch = torch.zeros((7, S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
+        scratch = torch.empty((4, S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
         pa.scratch = scratch.data_ptr()
         pa.tb_k = 0
     gmode = _native.GRAD_EXACT if exact else _native.GRAD_REFERENCE

@@ -532,7 +532,7 @@ def _adjoint_exact(G, b, er, se, mr, grad_output, want_er, want_se):
     NI, NJ = b.NI, b.NJ
     ge = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
     gs = torch.zeros((NI, NJ), dtype=torch.float64, device=dev)
-    scratch = torch.zeros((7, b.S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
+    scratch = torch.empty((4, b.S, NI + 1, b.pitch), dtype=torch.float32, device=dev)
     p = b.problem(coef, resid=resid, grad_eps=ge, grad_sig=gs)
     p.tb_k = 0
     p.scratch = scratch.data_ptr()

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define FGPR_ABI_VERSION 12
+#define FGPR_ABI_VERSION 13
 
 /* Error codes. */
 #define FGPR_OK 0
@@ -123,7 +123,7 @@ typedef struct fgpr_problem {
                                    (centre and halo) */
     double* gshot;              /* adjoint: S x 2 x NI x NJ per-shot FP64 partial gradients, shot-major
                                    (a shot range is a contiguous slice); zeroed by the caller */
-    float* scratch;             /* exact adjoint: 7*S*(NI+1)*pitch floats, zeroed by the caller */
+    float* scratch;             /* exact adjoint: 4*S*(NI+1)*pitch floats (no initialisation) */
     int32_t snap_base;          /* step of snapshot slot 0 (checkpointed segments; 0 otherwise) */
     int32_t nsnap;              /* snapshot slots in `snaps` (nt when it holds every step) */
     int32_t snap_mod;           /* > 0: step n lives in slot n % snap_mod (a ring of segment
