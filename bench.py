@@ -485,6 +485,9 @@ def multi_roofline(ub, reps):
                             "frac": round(t / d["us"] / 1e3 / HBM_PEAK_GBS, 4),
                             "what": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this "
                                     "kernel on this workload, copy-calibrated, / launch time"}
+        # the measured-bytes figures beside the algorithmic ones, at the same level
+        roof["achieved_physical"] = roof["physical"]["achieved"]
+        roof["frac_physical"] = roof["physical"]["frac"]
     return roof, {n: round(v["us"], 2) for n, v in per.items()}
 
 
@@ -579,6 +582,8 @@ def c2_line(dev, args, extras=True):
         us = roofline["avg_launch_us"]
         roofline["physical"] = {"achieved": round(t / us / 1e3, 1),
                                 "frac": round(t / us / 1e3 / HBM_PEAK_GBS, 4)}
+        roofline["achieved_physical"] = roofline["physical"]["achieved"]
+        roofline["frac_physical"] = roofline["physical"]["frac"]
     if tb_used and sweeps:
         blocks = -(-wl.nt // tb_used[2])
         per = {}
