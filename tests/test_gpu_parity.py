@@ -310,3 +310,41 @@ def test_graph_cache_stays_bounded(hip_device):
     finally:
         solver.set_graph_limit(prev)
         solver.release_graphs()
+
+
+class _PoisonedEmpty:
+    """The torch module with every float32 ``empty`` filled with NaN."""
+
+    def __init__(self, real):
+        self._real = real
+
+    def __getattr__(self, name):
+        return getattr(self._real, name)
+
+    def empty(self, *args, **kwargs):
+        t = self._real.empty(*args, **kwargs)
+        if t.dtype == self._real.float32:
+            t.fill_(float("nan"))
+        return t
+
+
+@pytest.mark.parametrize("checkpoint", [None, 40])
+def test_exact_gradient_reads_no_uninitialised_scratch(hip_device, monkeypatch, checkpoint):
+    """The exact adjoint's scratch (and every other float32 buffer the solver
+    takes uninitialised) is written before it is read: with each such buffer
+    filled with NaN first, the gradient is bitwise the usual one."""
+    import torch
+    from fastgpr import checkpoint as ck
+    from fastgpr import solver
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=160)
+    resid = _resid(wl)
+    print("case", wl.name, "checkpoint", checkpoint, flush=True)
+    _, ref = run_product(wl, hip_device, resid=resid, gradient="exact", checkpoint=checkpoint)
+    poisoned = _PoisonedEmpty(torch)
+    monkeypatch.setattr(solver, "torch", poisoned)
+    monkeypatch.setattr(ck, "torch", poisoned)
+    _, got = run_product(wl, hip_device, resid=resid, gradient="exact", checkpoint=checkpoint)
+    for k in ("er", "se"):
+        assert np.isfinite(got[k]).all()
+        assert np.array_equal(got[k], ref[k]), k
