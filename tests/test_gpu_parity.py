@@ -328,23 +328,29 @@ class _PoisonedEmpty:
         return t
 
 
-@pytest.mark.parametrize("checkpoint", [None, 40])
-def test_exact_gradient_reads_no_uninitialised_scratch(hip_device, monkeypatch, checkpoint):
-    """The exact adjoint's scratch (and every other float32 buffer the solver
-    takes uninitialised) is written before it is read: with each such buffer
-    filled with NaN first, the gradient is bitwise the usual one."""
+@pytest.mark.parametrize("gradient,time_block,checkpoint", [
+    ("exact", None, None), ("exact", None, 40), ("reference", None, None),
+    ("reference", (64, 1024, 6), None), ("reference", (64, 1024, 6), 40)])
+def test_gradient_reads_no_uninitialised_buffer(hip_device, monkeypatch, gradient, time_block,
+                                                checkpoint):
+    """Every float32 buffer the solver takes uninitialised (the exact
+    adjoint's scratch, snapshot histories and rings, ...) is written before
+    it is read: with each such buffer filled with NaN first, the traces and
+    the gradient are bitwise the usual ones."""
     import torch
     from fastgpr import checkpoint as ck
     from fastgpr import solver
     from fastgpr.workloads import small_case
     wl = small_case(nt=160)
     resid = _resid(wl)
-    print("case", wl.name, "checkpoint", checkpoint, flush=True)
-    _, ref = run_product(wl, hip_device, resid=resid, gradient="exact", checkpoint=checkpoint)
+    kw = dict(resid=resid, gradient=gradient, time_block=time_block, checkpoint=checkpoint)
+    print("case", wl.name, gradient, time_block, "checkpoint", checkpoint, flush=True)
+    tr_ref, ref = run_product(wl, hip_device, **kw)
     poisoned = _PoisonedEmpty(torch)
     monkeypatch.setattr(solver, "torch", poisoned)
     monkeypatch.setattr(ck, "torch", poisoned)
-    _, got = run_product(wl, hip_device, resid=resid, gradient="exact", checkpoint=checkpoint)
+    tr, got = run_product(wl, hip_device, **kw)
+    assert np.array_equal(tr, tr_ref)
     for k in ("er", "se"):
         assert np.isfinite(got[k]).all()
         assert np.array_equal(got[k], ref[k]), k
