@@ -130,14 +130,25 @@ def test_captured_schedule_equals_eager_ticks(hip_device):
                 assert torch.equal(v.cpu(), r[k]), (rep, k)
 
 
-def test_uneven_batch_waves_reuse_memory_and_match_compute(hip_device):
+@pytest.mark.parametrize("poison", [False, True])
+def test_uneven_batch_waves_reuse_memory_and_match_compute(hip_device, monkeypatch, poison):
     """A budget that holds one job's whole history but not all three: the
     jobs run in waves carved from one buffer (a wave reuses the memory of the
     one before) and a job too large alone keeps a shorter snapshot ring; every
-    result is bitwise compute()'s for that model alone, on every replay."""
+    result is bitwise compute()'s for that model alone, on every replay --
+    also with the carved buffers filled with NaN bytes first (``poison``), so
+    no job reads memory it has not written."""
+    from fastgpr import batching
     from fastgpr.batching import job_bytes
     from fastgpr.uneven import Problem, UnevenBatch, plan_waves
     dev = hip_device
+    if poison:
+        arena_init = batching.Arena.__init__
+
+        def nan_arena(self, device, nbytes):
+            arena_init(self, device, nbytes)
+            self.buf.fill_(0xFF)
+        monkeypatch.setattr(batching.Arena, "__init__", nan_arena)
     cases = _cases()
     obs = [torch.tensor(run_product(wl, dev, time_block=tb)[0]).to(dev) for wl, tb, _ in cases]
     inits = [(wl.er * 0.97, wl.se * 1.1) for wl, _, _ in cases]

@@ -162,3 +162,35 @@ def test_c5_grid_mix_alone_equals_mix(hip_device):
         e = rel_l2(r.result[k].cpu().numpy()[..., 0], ref)
         print(f"128x128 gradient {k} rel-L2 vs FP64 oracle {e:.3e}", flush=True)
         assert e <= 1e-4, k
+
+
+def test_mix_in_a_nan_filled_arena_equals_alone(hip_device):
+    """Arena memory is reused from request to request: with the whole arena
+    filled with NaN bytes before the first request, every result of a
+    staggered mix is finite and bitwise the one served alone, so no job
+    reads memory it has not written."""
+    from fastgpr.batching import Arena, job_bytes
+    from fastgpr.serving import Server
+    dev = hip_device
+    reqs = _requests(5, seed=23)
+    sizes = [job_bytes(r.wl.NI, r.wl.NJ, r.wl.S, r.wl.nt, r.wl.rx.shape[1], r.wl.P, r.wl.nt)
+             for r in reqs]
+    arena = Arena(dev, int(sum(sorted(sizes)[-2:])))   # blocks are reused as requests finish
+    arena.buf.fill_(0xFF)                              # every float32 word a NaN
+    srv = Server(dev, arena=arena)
+    for r in reqs[:2]:
+        srv.submit(r)
+    for _ in range(3):
+        srv.step()
+    for r in reqs[2:]:
+        srv.submit(r)
+        srv.step()
+    summary = srv.run()
+    torch.cuda.synchronize()
+    print(summary, flush=True)
+    assert summary["completed"] == len(reqs) and summary["refused"] == 0
+    for r in reqs:
+        a = _alone(r, dev)
+        for k in ("er", "se", "traces"):
+            assert torch.isfinite(r.result[k]).all(), (r.rid, k)
+            assert torch.equal(r.result[k], a.result[k]), (r.rid, k)
