@@ -14,6 +14,7 @@ the stopping test on the device, no host synchronisation per iteration);
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import torch
 
@@ -46,24 +47,28 @@ def total_variation(data: torch.Tensor, lamda: float = 0.02, rho: float = 1.0,
     from .solver import _stream
     data.squeeze_(-1)
     M, N = data.shape
-    x, work, out = _buffers(data.device, M, N)
-    x.copy_(data.detach())                    # float32, rounded as .to(torch.float32) rounds
-    nbytes = work.numel()
-    with _trace.span(f"fgpr tv {M}x{N}"):
-        _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
-                                          float(rho), int(num_iter), float(tol),
-                                          ctypes.c_void_p(work.data_ptr()), nbytes,
-                                          ctypes.c_void_p(out.data_ptr()), _stream(x.device)),
-                      "tv")
-    return out.clone().unsqueeze(-1)
+    with _BUFFER_LOCK:          # one caller at a time enqueues on the shared buffers
+        x, work, out = _buffers(data.device, M, N)
+        x.copy_(data.detach())                # float32, rounded as .to(torch.float32) rounds
+        nbytes = work.numel()
+        with _trace.span(f"fgpr tv {M}x{N}"):
+            _native.check(_native.lib.fgpr_tv(ctypes.c_void_p(x.data_ptr()), M, N, float(lamda),
+                                              float(rho), int(num_iter), float(tol),
+                                              ctypes.c_void_p(work.data_ptr()), nbytes,
+                                              ctypes.c_void_p(out.data_ptr()),
+                                              _stream(x.device)), "tv")
+        return out.clone().unsqueeze(-1)
 
 
 # Input, workspace and output buffers kept per (device, stream, grid): the
 # same pointers on every call, so the library replays the call's captured
 # graph instead of capturing a new one.  Calls on one stream are ordered, so
-# a call may reuse the buffers of the one before; the result is a copy.
+# a call may reuse the buffers of the one before; the result is a copy.  The
+# lock keeps two threads on one stream from interleaving their copy-in,
+# launches and copy-out on the same buffers.
 _BUFFERS: dict = {}
 _BUFFER_SETS = 8
+_BUFFER_LOCK = threading.Lock()
 
 
 def _buffers(device, M: int, N: int):
