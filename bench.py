@@ -1030,7 +1030,8 @@ def plain_path(dev, value):
             pred, _ = fastgpr.compute(*common, er, se, mr, 1, wl.rx.shape[1],
                                       time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P,
                                       tv=False, customsource=src[s:s + 1],
-                                      customreceiver=rx[s:s + 1], exec_mode="eager")
+                                      customreceiver=rx[s:s + 1], exec_mode="eager",
+                                      time_block=None)
             (0.5 * ((pred - obs[s:s + 1]) ** 2).sum()).backward()
 
     one_gradient()

@@ -30,6 +30,11 @@ CANDIDATES = ([(32, 256, k) for k in (1, 2, 3, 4, 6)]
 _cache: dict = {}
 
 
+class NoVariant(RuntimeError):
+    """No compiled time-blocked variant runs this grid and acquisition (for
+    example more receivers in one region than a region's lists hold)."""
+
+
 def _key(G, candidates=None) -> str:
     name = torch.cuda.get_device_name(G.device)
     groups = max(1, min(int(G.options.get("shot_streams", 3) or 1), G.S, _solver.max_shot_groups()))
@@ -152,8 +157,8 @@ def choose(G, er, se, mr, candidates=None) -> tuple:
             errors.append(f"{tb}: {e}")
             continue
     if not results:
-        raise RuntimeError(f"no time-blocked variant runs on a {G.NI} x {G.NJ} grid with "
-                           f"{G.S} shots: " + "; ".join(errors[:4]))
+        raise NoVariant(f"no time-blocked variant runs on a {G.NI} x {G.NJ} grid with "
+                        f"{G.S} shots: " + "; ".join(errors[:4]))
     best = min(results, key=results.get)
     _cache[key] = best
     _save_file()

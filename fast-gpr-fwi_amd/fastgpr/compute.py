@@ -13,9 +13,13 @@ Extra keyword arguments (all default to the reference behaviour):
 * ``graph_chunk`` -- steps per captured graph (default: 32 for the per-step
                      kernels and checkpointed segments, 80 launches for a
                      full-history time-blocked sweep).
-* ``time_block``  -- None (one H and one E kernel per step), "auto" (the
-                     autotuner picks a time-blocked variant for this grid) or
-                     an explicit (region rows, threads, steps per launch).
+* ``time_block``  -- "auto" (the default: the autotuner picks a time-blocked
+                     variant for this grid; where no variant holds the
+                     geometry, a warning and the per-step kernels), None (one
+                     H and one E kernel per step) or an explicit (region
+                     rows, threads, steps per launch).  Traces are bitwise the
+                     same every way; gradients agree to ~1e-12 (FP64 sums
+                     blocked differently).
 * ``gradient``    -- "reference" (the reference's normalised approximate
                      gradient) or "exact" (dL/d er, dL/d se of the discrete
                      forward map through its exact adjoint, unnormalised).
@@ -40,6 +44,7 @@ from __future__ import annotations
 
 import atexit
 import time
+import warnings
 
 import torch
 from torch import Tensor
@@ -123,7 +128,7 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
             receiver_location=None, receiver_step=None, pmlthick=0, freq=None,
             dt=None, pml_formulation='HORIPML', total_step=None, tv=True,
             Fixedreceiver=False, nseg=None, customsource=None, customreceiver=None,
-            exec_mode="graph", graph_chunk=None, time_block=None, gradient="reference",
+            exec_mode="graph", graph_chunk=None, time_block="auto", gradient="reference",
             checkpoint="auto", shot_streams=3):
     """Forward-model ``step`` shots; returns (traces (step, nt, nrx), dt).
 
@@ -142,8 +147,12 @@ def compute(device, dx, ny, nx, nz, source, source_apmlitudes, step, er, se, mr,
                        exec_mode=exec_mode, graph_chunk=graph_chunk, time_block=time_block,
                        gradient=gradient, checkpoint=checkpoint, shot_streams=shot_streams)
     if G.time_block == "auto":
-        from .autotune import choose
-        G.time_block = choose(G, er, se, mr)
+        from .autotune import NoVariant, choose
+        try:
+            G.time_block = choose(G, er, se, mr)
+        except NoVariant as e:
+            warnings.warn(f"{e}; using the per-step kernels", RuntimeWarning, stacklevel=2)
+            G.time_block = None
     rxs = GPRFWI.apply(G, er, se, mr)
     return rxs, G.dt
 

@@ -354,3 +354,62 @@ def test_gradient_reads_no_uninitialised_buffer(hip_device, monkeypatch, gradien
     for k in ("er", "se"):
         assert np.isfinite(got[k]).all()
         assert np.array_equal(got[k], ref[k]), k
+
+
+def _compute_default(wl, dev, er, se, mr):
+    """fastgpr.compute with the reference's arguments only (no build extras)."""
+    import torch
+    import fastgpr
+    return fastgpr.compute(dev, wl.dx, wl.NJ, wl.NI, 1, ['hertzian', 'z'], ['ricker', 1, wl.freq],
+                           wl.S, er, se, mr, wl.src.shape[1], wl.rx.shape[1],
+                           time_windows=wl.time_window, dt=wl.dt, pmlthick=wl.P, tv=False,
+                           customsource=torch.tensor(wl.src),
+                           customreceiver=torch.tensor(wl.rx))[0]
+
+
+def _default_vs_per_step(wl, dev):
+    import torch
+    from helpers import product_tensors
+    resid = torch.tensor(_resid(wl)).to(dev)
+    er, se, mr = product_tensors(wl, dev)
+    er.requires_grad_(True)
+    se.requires_grad_(True)
+    tr = _compute_default(wl, dev, er, se, mr)
+    tr.backward(resid)
+    torch.cuda.synchronize()
+    ref_tr, ref_g = run_product(wl, dev, resid=resid.cpu().numpy())      # time_block=None
+    return (tr.detach().cpu().numpy(), er.grad.cpu().numpy()[..., 0],
+            se.grad.cpu().numpy()[..., 0], ref_tr, ref_g)
+
+
+def test_default_call_runs_autotuned_time_blocked_kernels(hip_device):
+    """compute() called with the reference's arguments only takes the
+    autotuned time-blocked path: one new tuner entry, traces bitwise the
+    per-step kernels', gradients within 1e-12."""
+    from fastgpr import autotune
+    from fastgpr.workloads import small_case
+    wl = small_case(nt=200)
+    before = set(autotune._cache)
+    tr, ge, gs, ref_tr, ref_g = _default_vs_per_step(wl, hip_device)
+    new = set(autotune._cache) - before
+    print("tuner entries added", new, flush=True)
+    assert len(new) == 1
+    assert np.array_equal(tr, ref_tr)
+    assert rel_l2(ge, ref_g["er"]) < 1e-12 and rel_l2(gs, ref_g["se"]) < 1e-12
+
+
+def test_default_call_falls_back_to_per_step_kernels(hip_device):
+    """200 receivers on a 20 x 10 block: no time-blocked variant's region
+    lists hold them, so the default call warns and runs the per-step
+    kernels, bitwise."""
+    from fastgpr.workloads import Workload, small_case
+    base = small_case(NI=40, NJ=40, P=5, S=1, nt=120, nrx=6, seed=31)
+    rx = np.zeros((1, 200, 3), np.int32)
+    rx[0, :, 0] = 10 + np.arange(200) // 10
+    rx[0, :, 1] = 10 + np.arange(200) % 10
+    wl = Workload(base.name + "_dense_rx", base.er, base.se, base.mr, base.er0, base.se0, base.dx,
+                  base.dt, base.nt, base.P, base.freq, base.src, rx)
+    with pytest.warns(RuntimeWarning, match="per-step kernels"):
+        tr, ge, gs, ref_tr, ref_g = _default_vs_per_step(wl, hip_device)
+    assert np.array_equal(tr, ref_tr)
+    assert np.array_equal(ge, ref_g["er"]) and np.array_equal(gs, ref_g["se"])
