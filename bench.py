@@ -81,6 +81,9 @@ def parse(argv=None):
     ap.add_argument("--rate", type=float, default=20.0, help="--serve arrival rate (requests/s)")
     ap.add_argument("--serve-histories", default="strict", choices=["share", "whole", "strict"],
                     help="the server's snapshot-history admission rule")
+    ap.add_argument("--serve-max-hold", type=float, default=2.0, metavar="SECONDS",
+                    help="the server's max_hold_s: how long a request waits before it holds back "
+                         "younger ones")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then "
                          "WRITE_SIZE; CSV output) into profiles/traffic.json; uses no GPU")
@@ -225,7 +228,8 @@ def main():
         tuner_envelope(dev)
         return
     if args.serve > 0:
-        print(json.dumps(serve_bench(dev, args.serve, args.rate, args.serve_histories)), flush=True)
+        print(json.dumps(serve_bench(dev, args.serve, args.rate, args.serve_histories,
+                                     args.serve_max_hold)), flush=True)
         return
     if args.workload == "c2" and args.section is None:
         out = c2_line(dev, args)
@@ -248,7 +252,8 @@ def main():
     elif sec == "c4":
         out = c4_memory_bound(dev)
     else:
-        out = serve_bench(dev, args.serve_check, args.serve_rate, args.serve_histories)
+        out = serve_bench(dev, args.serve_check, args.serve_rate, args.serve_histories,
+                          args.serve_max_hold)
     print(json.dumps(out), flush=True)
 
 
@@ -1046,7 +1051,7 @@ def plain_path(dev, value):
             "what": "per-step eager launches, one shot per launch sequence, no graph"}
 
 
-def serve_bench(dev, seconds, rate, histories="strict"):
+def serve_bench(dev, seconds, rate, histories="strict", max_hold_s=2.0):
     """Serving loop under Poisson arrivals: requests (grid uniform over
     128x128, 256x128, 512x256, 1024x512; 1-32 shots; 1500-5000 steps by grid
     diagonal) are submitted from a thread at their arrival times, admitted by
@@ -1079,7 +1084,7 @@ def serve_bench(dev, seconds, rate, histories="strict"):
 
     del warm
     torch.cuda.empty_cache()
-    srv = Server(dev, histories=histories)
+    srv = Server(dev, histories=histories, max_hold_s=max_hold_s)
     t0 = time.perf_counter()
 
     def submitter():
